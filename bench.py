@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark of the split-operator hot path on one MI355X.
+
+Workload (the headline configuration): 2D split-operator evolution, O2
+(Strang), 1024 x 1024 periodic grid on [-150,150)^2, dt = 0.2, complex128;
+potential = 16 seeded Gaussian bumps plus an imaginary absorbing layer;
+Gaussian packet sigma=8 at (-60,0), k=(0.8,0).  One "step" = one split step
+of the whole grid.  psi, V and the phase tables are resident in HBM before
+the timed region starts.
+
+Prints one JSON line: grid-pt-steps/s, the per-kernel HBM roofline of the
+dominant kernel (algorithmic bytes / HIP-event-measured launch duration) and
+the CPU baseline (the FP64 oracle port of the reference CpuSeq+KISS path timed
+on this host) -- plus the FP32-mixed rate and error on the 2048^2 form."""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("2D split-op grid-pt-steps/s FP64 vs FP32 @L2err≤1e-5; "
+          "served solves/s + p99")
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per grid point per launch (complex128 psi read + write,
+# plus the potential factor read by the row sweep)
+ROW_B_PER_PT = {64: 48.0, 32: 32.0}   # FP32-mixed reads an FP64 (hi+lo) factor
+COL_B_PER_PT = {64: 32.0, 32: 16.0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_job(N, prob, precision):
+    from qusim_amd import _native as Nn
+    n = prob["n"]
+    psi = Nn.DeviceArray.from_host(prob["psi"])
+    vre = Nn.DeviceArray.from_host(np.ascontiguousarray(prob["V"].real))
+    vim = Nn.DeviceArray.from_host(np.ascontiguousarray(prob["V"].imag))
+    d = Nn.SplitDesc()
+    d.dim, d.nx, d.ny = 2, n, n
+    d.x0, d.x1, d.y0, d.y1 = prob["x0"], prob["x1"], prob["x0"], prob["x1"]
+    d.dt_re, d.dt_im = prob["dt"], 0.0
+    d.mass = d.hbar = 1.0
+    d.method, d.boundary, d.precision = 2, 0, precision
+    p = ctypes.c_void_p()
+    Nn.call("qsim_hip_split_create", ctypes.byref(p), ctypes.byref(d), vre.ptr, vim.ptr, psi.ptr)
+    return p.value, (psi, vre, vim)
+
+
+def time_steps(job, steps, warmup):
+    from qusim_amd import _native as Nn
+    if warmup:
+        Nn.call("qsim_hip_split_steps", job, warmup, None)
+    Nn.synchronize()
+    ev0, ev1 = Nn.Event(), Nn.Event()
+    t0 = time.perf_counter()
+    ev0.record()
+    Nn.call("qsim_hip_split_steps", job, steps, None)
+    ev1.record()
+    Nn.synchronize()
+    t1 = time.perf_counter()
+    return t1 - t0, ev0.elapsed_ms(ev1) / 1e3
+
+
+def kernel_times(job, steps):
+    from qusim_amd import _native as Nn
+    ms = (ctypes.c_double * 3)()
+    cnt = (ctypes.c_int * 3)()
+    Nn.call("qsim_hip_split_steps_timed", job, steps, ms, cnt, None)
+    return list(ms), list(cnt)
+
+
+def roofline(ms, cnt, npts, precision):
+    per = {}
+    for k, name, bpp in ((0, "row_sweep", ROW_B_PER_PT[precision]),
+                         (1, "col_sweep", COL_B_PER_PT[precision])):
+        if cnt[k]:
+            avg_s = ms[k] / cnt[k] / 1e3
+            gbs = bpp * npts / avg_s / 1e9
+            per[name] = {"avg_us": avg_s * 1e6, "launches": cnt[k], "bytes_per_launch": bpp * npts,
+                         "achieved_GBs": gbs, "total_ms": ms[k]}
+    dom = max(per, key=lambda k: per[k]["total_ms"])
+    return dom, per
+
+
+def cpu_baseline(prob, steps):
+    from oracle import pyoracle as O
+    p = prob
+    secs = O.split2d_time(p["psi"], p["V"], p["x0"], p["x1"], p["x0"], p["x1"], p["dt"], steps,
+                          complex_v=True, threads=1)
+    n = p["n"]
+    return steps * n * n / secs, secs
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(name)
+    except Exception:
+        return None
+
+
+def fp32_section(steps_err, steps_rate):
+    """FP32-mixed 2048^2 (dt=0.1): rate, and error vs the complex128 path of
+    this device after `steps_err` steps (the oracle's parity with the FP64
+    device path is established by the test suite)."""
+    from qusim_amd import _native as Nn
+    from qusim_amd.workloads import c3_problem
+    prob = c3_problem()
+    n = prob["n"]
+    j64, keep64 = make_job(Nn, prob, 64)
+    j32, keep32 = make_job(Nn, prob, 32)
+    wall, gpu = time_steps(j32, steps_rate, 2)
+    rate32 = steps_rate * n * n / gpu
+    w64, g64 = time_steps(j64, steps_rate, 2)
+    rate64 = steps_rate * n * n / g64
+    ms, cnt = kernel_times(j32, 4)
+    dom, per = roofline(ms, cnt, n * n, 32)
+    # error after steps_err steps from the same initial state
+    Nn.call("qsim_hip_memcpy_h2d", keep64[0].ptr, np.ascontiguousarray(prob["psi"]).ctypes.data,
+            keep64[0].nbytes)
+    Nn.call("qsim_hip_memcpy_h2d", keep32[0].ptr, np.ascontiguousarray(prob["psi"]).ctypes.data,
+            keep32[0].nbytes)
+    Nn.call("qsim_hip_split_steps", j64, steps_err, None)
+    Nn.call("qsim_hip_split_steps", j32, steps_err, None)
+    a = keep64[0].download()
+    b = keep32[0].download()
+    err = float(np.linalg.norm(b - a) / np.linalg.norm(a))
+    dx = (prob["x1"] - prob["x0"]) / n
+    n64 = float(np.sum(np.abs(a) ** 2) * dx * dx)
+    n32 = float(np.sum(np.abs(b) ** 2) * dx * dx)
+    for j in (j64, j32):
+        Nn.call("qsim_hip_split_destroy", j)
+    return {"workload": "2D O2 2048x2048, dt=0.1, FP32 data + FP64 factors",
+            "value": rate32, "unit": "grid-pt-steps/s", "fp64_value_same_grid": rate64,
+            "speedup_vs_fp64": rate32 / rate64, "psi_rel_l2_vs_fp64": err,
+            "err_steps": steps_err, "norm_drift_vs_fp64": (n32 - n64) / n64,
+            "dominant_kernel": dom, "kernels": per}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--fp32-err-steps", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(every box here has one GPU; see DESIGN.md for how the grid would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from qusim_amd import _native as Nn
+    from qusim_amd.workloads import c2_problem
+    Nn.init(0)
+    info = Nn.device_info()
+    log("device: %s, %d CUs" % (info["name"], info["cu"]))
+    prob = c2_problem()
+    n = prob["n"]
+    npts = n * n
+    job, keep = make_job(Nn, prob, 64)
+    log("timing %d steps of 1024^2 (warmup %d)" % (args.steps, args.warmup))
+    wall, gpu = time_steps(job, args.steps, args.warmup)
+    value = args.steps * npts / wall
+    log("C2 FP64: %.3e grid-pt-steps/s (%.3f ms/step)" % (value, wall / args.steps * 1e3))
+    ms, cnt = kernel_times(job, max(4, args.steps // 4))
+    dom, per = roofline(ms, cnt, npts, 64)
+    achieved = per[dom]["achieved_GBs"]
+    step_gbs = 80.0 * npts / (gpu / args.steps) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-pt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D split-operator O2 step, 1024x1024 periodic grid on "
+                               "[-150,150)^2, dt=0.2, complex128, 16 seeded Gaussian bumps "
+                               "+ imaginary absorbing layer",
+                   "grid": [n, n], "precision": "complex128", "method": "Strang O2",
+                   "device": info["name"]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(dom),
+                     "step_algorithmic_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
+                     "kernels": per},
+    }
+    Nn.call("qsim_hip_split_destroy", job)
+    if not args.no_fp32:
+        log("FP32-mixed 2048^2 section")
+        out["fp32_mixed"] = fp32_section(args.fp32_err_steps, 50)
+    if not args.no_cpu:
+        log("CPU baseline: %d steps of the oracle on 1 thread" % args.cpu_steps)
+        rate, secs = cpu_baseline(prob, args.cpu_steps)
+        out["cpu_baseline"] = {"value": rate, "unit": "grid-pt-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "%d O2 steps of the same 1024^2 complex-V workload "
+                                         "(%.1f s), FP64 oracle port of CpuSeq+KISS, 1 thread"
+                                         % (args.cpu_steps, secs)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

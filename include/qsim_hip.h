@@ -1,0 +1,278 @@
+/*
+ * qsim_hip.h -- C ABI of libqsim_hip.so, the MI355X (gfx950) device behind
+ * QuSim's Device / FourierTransform plug-in boundary.
+ *
+ * Plain C: pointers, sizes, doubles.  Every array argument is a DEVICE
+ * pointer obtained from qsim_hip_malloc (or any HIP allocation on the current
+ * device) unless the comment says "host".  Complex numbers are interleaved
+ * (re, im), the layout of std::complex<double> / std::complex<float>.
+ * Arrays are row-major, last axis contiguous: idx = i*ny + j (2D),
+ * idx = (i*ny + j)*nz + k (3D)  (reference include/QuSim.h:47-61).
+ *
+ * Every function returns QSIM_OK (0) or an error code; the message of the
+ * last error on the calling thread is qsim_hip_last_error().  A host adapter
+ * turns a non-zero status into std::runtime_error, as the reference's
+ * backends throw (src/CudaDevice.cu:31).
+ *
+ * Work is enqueued on `stream` (a hipStream_t; NULL = the null stream).
+ * Reductions and element reads return their result to the host and
+ * therefore synchronise that stream, as the reference's device ops are
+ * synchronous (src/CudaDevice.cu:129-302).
+ */
+#ifndef QSIM_HIP_H
+#define QSIM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSIM_OK 0
+#define QSIM_ERR_INVALID 1      /* bad argument (size, pointer, option)       */
+#define QSIM_ERR_HIP 2          /* HIP runtime error                          */
+#define QSIM_ERR_NOMEM 3        /* device allocation failed                   */
+#define QSIM_ERR_UNSUPPORTED 4  /* valid request this build does not support  */
+
+typedef struct { double re, im; } qsim_c128;
+typedef struct { float re, im; } qsim_c64;
+typedef void *qsim_stream_t;
+
+/* ---------------------------------------------------------------- runtime */
+const char *qsim_hip_last_error(void);
+int qsim_hip_init(int device);                /* select device, set up library state */
+int qsim_hip_device_info(char *name, size_t name_len, int *cu_count,
+                         size_t *mem_free, size_t *mem_total);
+int qsim_hip_synchronize(qsim_stream_t stream);
+int qsim_hip_stream_create(qsim_stream_t *stream);
+int qsim_hip_stream_destroy(qsim_stream_t stream);
+
+/* Device::AllocMem / Free / MemToDevice / MemToHost (src/Device.h:53-80).
+ * Copies are synchronous with respect to the host. */
+int qsim_hip_malloc(void **ptr, size_t bytes);
+int qsim_hip_free(void *ptr);
+int qsim_hip_memcpy_h2d(void *dst, const void *host_src, size_t bytes);
+int qsim_hip_memcpy_d2h(void *host_dst, const void *src, size_t bytes);
+int qsim_hip_memcpy_d2d(void *dst, const void *src, size_t bytes, qsim_stream_t stream);
+
+/* Timing with HIP events on a stream (for measurement harnesses). */
+int qsim_hip_event_create(void **ev);
+int qsim_hip_event_destroy(void *ev);
+int qsim_hip_event_record(void *ev, qsim_stream_t stream);
+int qsim_hip_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ------------------------------------------------------- pointwise Device ops
+ * Each replaces the DeviceTemplate<CudaExec> method of the same meaning
+ * (src/Device.h:53-173, functors src/DeviceTemplate.h:531-1134). */
+int qsim_hip_set_zero(qsim_c128 *a, size_t n, qsim_stream_t s);             /* SetZero  :1185 */
+int qsim_hip_set_one(qsim_c128 *a, size_t n, qsim_stream_t s);              /* SetOne   :1192 */
+int qsim_hip_copy(qsim_c128 *to, const qsim_c128 *from, size_t n, qsim_stream_t s); /* Copy :1313 */
+int qsim_hip_copy_real(double *to, const double *from, size_t n, qsim_stream_t s);  /* Copy :1321 */
+/* Scale(DComplex*, double) :1345 ; Scale(DComplex*, DComplex) :1353 */
+int qsim_hip_scale(qsim_c128 *a, double scale, size_t n, qsim_stream_t s);
+int qsim_hip_scale_c(qsim_c128 *a, qsim_c128 scale, size_t n, qsim_stream_t s);
+/* Scale(to, from, double) :1370 */
+int qsim_hip_scale_to(qsim_c128 *to, const qsim_c128 *from, double scale, size_t n,
+                      qsim_stream_t s);
+/* Mul(to, from2): to[i] = from2[i]*to[i]  :1458 ; Mul(to, a, b) :1414 */
+int qsim_hip_mul(qsim_c128 *to, const qsim_c128 *from, size_t n, qsim_stream_t s);
+int qsim_hip_mul3(qsim_c128 *to, const qsim_c128 *a, const qsim_c128 *b, size_t n,
+                  qsim_stream_t s);
+/* Exp(to, DReal* V, alpha): to = exp(alpha*V)  :1294 ; complex V :1312 */
+int qsim_hip_exp_real(qsim_c128 *to, const double *V, qsim_c128 alpha, size_t n,
+                      qsim_stream_t s);
+int qsim_hip_exp_complex(qsim_c128 *to, const qsim_c128 *V, qsim_c128 alpha, size_t n,
+                         qsim_stream_t s);
+/* MulExp(psi, alpha, DReal* V): psi *= exp(alpha*V)  :1488 ; complex V :1479 */
+int qsim_hip_mul_exp_real(qsim_c128 *psi, qsim_c128 alpha, const double *V, size_t n,
+                          qsim_stream_t s);
+int qsim_hip_mul_exp_complex(qsim_c128 *psi, qsim_c128 alpha, const qsim_c128 *V, size_t n,
+                             qsim_stream_t s);
+/* MulExpK1D/2D/3D: psi *= exp(alpha * sum_axis a_axis * fold(i)^2)  :1561-1589.
+ * K1D takes the product alpha*a as one complex (reference signature). */
+int qsim_hip_mul_exp_k1d(qsim_c128 *psi, qsim_c128 alpha, size_t n, qsim_stream_t s);
+int qsim_hip_mul_exp_k2d(qsim_c128 *psi, qsim_c128 alpha, double ax, size_t nx, double ay,
+                         size_t ny, qsim_stream_t s);
+int qsim_hip_mul_exp_k3d(qsim_c128 *psi, qsim_c128 alpha, double ax, size_t nx, double ay,
+                         size_t ny, double az, size_t nz, qsim_stream_t s);
+/* MulExpIdx2: psi[i] *= exp(alpha * i^2), i < n  :1520 */
+int qsim_hip_mul_exp_idx2(qsim_c128 *psi, qsim_c128 alpha, size_t n, qsim_stream_t s);
+/* CopyReverseMinu(out, in, n): out[-m] = -in[m], m < n  :1303 (out points at the
+ * LAST element written) */
+int qsim_hip_copy_reverse_minus(qsim_c128 *out_last, const qsim_c128 *in, size_t n,
+                                qsim_stream_t s);
+/* Set / At single elements :1168-1180, :1155-1166 (At is synchronous) */
+int qsim_hip_set_elem(qsim_c128 *a, size_t idx, qsim_c128 v, qsim_stream_t s);
+int qsim_hip_get_elem(const qsim_c128 *a, size_t idx, qsim_c128 *host_out, qsim_stream_t s);
+
+/* The remaining Device methods (src/Device.h:53-173) */
+int qsim_hip_exp_i_real(qsim_c128 *to, const double *V, double alpha, size_t n,
+                        qsim_stream_t s);                                      /* ExpI :1303 */
+int qsim_hip_sub(qsim_c128 *to, const qsim_c128 *a, const qsim_c128 *b, size_t n,
+                 qsim_stream_t s);                                             /* Sub :1383 */
+int qsim_hip_mul_r(qsim_c128 *to, const qsim_c128 *a, const qsim_c128 *b, size_t n,
+                   qsim_stream_t s);                                           /* MulR :1393 */
+int qsim_hip_mul_minus_i(qsim_c128 *to, const qsim_c128 *a, const qsim_c128 *b, size_t n,
+                         qsim_stream_t s);                                     /* MulMinusI :1403 */
+int qsim_hip_mul_real_c(qsim_c128 *to, const double *a, const qsim_c128 *b, size_t n,
+                        qsim_stream_t s);                                      /* Mul(DReal*) :1436 */
+int qsim_hip_mul_real_alpha_c(qsim_c128 *to, const double *a, qsim_c128 alpha,
+                              const qsim_c128 *b, size_t n, qsim_stream_t s); /* Mul(DReal*,a) :1446 */
+int qsim_hip_mul_exp3(qsim_c128 *to, const qsim_c128 *a, qsim_c128 alpha, const qsim_c128 *b,
+                      size_t n, qsim_stream_t s);                              /* MulExp(to,..) :1468 */
+int qsim_hip_mul_exp_i_real(qsim_c128 *psi, double alpha, const double *V, size_t n,
+                            qsim_stream_t s);                                  /* MulExpI :1498 */
+int qsim_hip_scale_real(double *a, double scale, size_t n, qsim_stream_t s);   /* Scale(DReal*) */
+int qsim_hip_scale_real_to(double *to, const double *from, double scale, size_t n,
+                           qsim_stream_t s);
+int qsim_hip_scale_c_to(qsim_c128 *to, const qsim_c128 *from, qsim_c128 scale, size_t n,
+                        qsim_stream_t s);
+int qsim_hip_set_elem_real(double *a, size_t idx, double v, qsim_stream_t s);
+int qsim_hip_get_elem_real(const double *a, size_t idx, double *host_out, qsim_stream_t s);
+
+/* Born-series pointwise ops */
+int qsim_hip_g01d(qsim_c128 *psik, double E, double eps, double a, size_t n, qsim_stream_t s);
+int qsim_hip_g02d(qsim_c128 *psik, double E, double eps, double a1, double a2, size_t nx,
+                  size_t ny, qsim_stream_t s);                                  /* G02D :1622 */
+int qsim_hip_g03d(qsim_c128 *psik, double E, double eps, double a1, double a2, double a3,
+                  size_t nx, size_t ny, size_t nz, qsim_stream_t s);            /* G03D :1635 */
+int qsim_hip_add_mul(qsim_c128 *psi, const qsim_c128 *psi0, const qsim_c128 *V, size_t n,
+                     qsim_stream_t s);                                          /* AddMul :1667 */
+int qsim_hip_add3(qsim_c128 *out, const qsim_c128 *dpsi, const qsim_c128 *psi0,
+                  const qsim_c128 *V, double eps, size_t n, qsim_stream_t s);   /* Add3 :1689 */
+int qsim_hip_linear_update(qsim_c128 *psi, const qsim_c128 *newpsi, double gamma, size_t n,
+                           qsim_stream_t s);                                    /* :1700 */
+int qsim_hip_vellekoop(qsim_c128 *psi, const qsim_c128 *newpsi, const qsim_c128 *V,
+                       double slow, double eps, size_t n, qsim_stream_t s);     /* :1710 */
+int qsim_hip_hao1(qsim_c128 *psi, const qsim_c128 *newpsi, const qsim_c128 *V, double slow,
+                  double eps, size_t n, qsim_stream_t s);                       /* :1720 */
+int qsim_hip_hao2(qsim_c128 *psi, const qsim_c128 *newpsi, const qsim_c128 *V, double slow,
+                  double eps, size_t n, qsim_stream_t s);                       /* :1730 */
+
+/* ------------------------------------------------------------ reductions
+ * FP64 compensated (double-double) accumulation, deterministic order. */
+int qsim_hip_norm2(const qsim_c128 *a, size_t n, double *out, qsim_stream_t s);     /* :1222 */
+int qsim_hip_norm2_real(const double *a, size_t n, double *out, qsim_stream_t s);   /* :1284 */
+int qsim_hip_minus_norm2(const qsim_c128 *a, const qsim_c128 *b, size_t n, double *out,
+                         qsim_stream_t s);                                          /* :1273 */
+int qsim_hip_max_abs(const qsim_c128 *a, size_t n, double *out, qsim_stream_t s);   /* Max :1241 */
+int qsim_hip_min_abs(const qsim_c128 *a, size_t n, double *out, qsim_stream_t s);   /* Min :1231 */
+int qsim_hip_sum_real(const qsim_c128 *a, size_t n, double *out, qsim_stream_t s);  /* :1251 */
+int qsim_hip_sum_imag(const qsim_c128 *a, size_t n, double *out, qsim_stream_t s);  /* :1262 */
+int qsim_hip_abs2_idx(const qsim_c128 *a, size_t n, double *out, qsim_stream_t s);  /* :1200 */
+int qsim_hip_abs2_idx2(const qsim_c128 *a, size_t n, double *out, qsim_stream_t s); /* :1529 */
+int qsim_hip_abs2_idx_real(const double *a, size_t n, double *out, qsim_stream_t s); /* :1210 */
+int qsim_hip_abs2_k1d(const qsim_c128 *a, double ax, size_t n, double *out,
+                      qsim_stream_t s);                                             /* :1539 */
+int qsim_hip_abs2_k2d(const qsim_c128 *a, double ax, size_t nx, double ay, size_t ny,
+                      double *out, qsim_stream_t s);                                /* :1550 */
+int qsim_hip_abs2_k3d(const qsim_c128 *a, double ax, size_t nx, double ay, size_t ny,
+                      double az, size_t nz, double *out, qsim_stream_t s);          /* :1561 */
+int qsim_hip_abs2_mul_real(const qsim_c128 *a, const double *V, size_t n, double *out,
+                           qsim_stream_t s);                                        /* :1601 */
+int qsim_hip_abs2_mul_complex(const qsim_c128 *a, const qsim_c128 *V, size_t n,
+                              qsim_c128 *out, qsim_stream_t s);                     /* :1591 */
+/* Xsection3D: sum (psi0+psi) Re V exp(-i k.r)  :1646 */
+int qsim_hip_xsection3d(const qsim_c128 *psi0, const qsim_c128 *psi, const qsim_c128 *V,
+                        double kx, double ky, double kz, double x0, double y0, double z0,
+                        double dx, double dy, double dz, size_t nx, size_t ny, size_t nz,
+                        qsim_c128 *out, qsim_stream_t s);
+/* 2D counterpart, replacing the host loop of src/ScatteringSolverImpl.cpp:186-191 */
+int qsim_hip_xsection2d(const qsim_c128 *psi0, const qsim_c128 *psi, const qsim_c128 *V,
+                        double kx, double ky, double x0, double y0, double dx, double dy,
+                        size_t nx, size_t ny, qsim_c128 *out, qsim_stream_t s);
+
+/* ------------------------------------------------------------------ FFT
+ * FourierTransform{1,2,3}D::Create / Transform (src/FourierTransform.h:12-36):
+ * unnormalised, forward sign -, inverse sign +, out of place (in == out is
+ * also allowed).  dims outer -> inner.  precision 64: complex128 data;
+ * precision 32: complex64 data (twiddles built in FP64, then rounded). */
+typedef struct qsim_fft_plan qsim_fft_plan;
+int qsim_hip_fft_plan_create(qsim_fft_plan **plan, int rank, const size_t *dims, int inverse,
+                             int precision);
+int qsim_hip_fft_exec(qsim_fft_plan *plan, const void *in, void *out, qsim_stream_t s);
+int qsim_hip_fft_plan_destroy(qsim_fft_plan *plan);
+
+/* ------------------------------------------------- fused split-operator job
+ * SplittingMethod1D/2D::UpdatePsi (src/SplittingUtils.h:14-64,
+ * src/SplittingMethod1D.cpp, src/SplittingMethod2D.cpp) as fused sweeps:
+ * each FFT is fused with the phase multiply that follows it, so a 2D step
+ * is two read+write passes over psi. */
+typedef struct {
+    int dim;              /* 1 or 2                                          */
+    size_t nx, ny;        /* ny = 1 for 1D                                   */
+    double x0, x1, y0, y1;
+    double dt_re, dt_im;  /* complex dt as the reference (include/QuSim.h:558) */
+    double mass, hbar;
+    int method;           /* 2 = SplittingMethodO2, 4 = SplittingMethodO4    */
+    int boundary;         /* 0 = Period, 1 = InfiniteWall (1D)               */
+    int precision;        /* 64 = complex128; 32 = FP32-mixed                */
+} qsim_split_desc;
+
+typedef struct qsim_split_job qsim_split_job;
+/* V_re: device, n doubles (the reference's real potential fV).  V_im: device,
+ * n doubles or NULL (complex-potential extension; NULL reproduces the
+ * reference evolver, which keeps Re V only, src/EvolverImpl2D.cpp:55).
+ * psi: device complex128 state, updated in place by qsim_hip_split_steps. */
+int qsim_hip_split_create(qsim_split_job **job, const qsim_split_desc *d, const double *V_re,
+                          const double *V_im, qsim_c128 *psi);
+/* nsteps split steps (QuUpdatePsi with fBatchSize = nsteps); psi is in real
+ * space and up to date when the enqueued work completes. */
+int qsim_hip_split_steps(qsim_split_job *job, size_t nsteps, qsim_stream_t s);
+/* As qsim_hip_split_steps, timing every kernel launch with HIP events on
+ * `stream`: kernel_ms[k] / kernel_launches[k] for k = 0 row sweeps, 1 column
+ * sweeps, 2 the persistent 1D kernel.  Synchronous.  Fused jobs only. */
+int qsim_hip_split_steps_timed(qsim_split_job *job, size_t nsteps, double *kernel_ms,
+                               int *kernel_launches, qsim_stream_t s);
+/* Re-read psi after the caller changed it (e.g. renormalised it). */
+int qsim_hip_split_load_psi(qsim_split_job *job, qsim_stream_t s);
+/* sum |psi|^2 over the job's working state (compensated FP64), synchronous */
+int qsim_hip_split_norm2(qsim_split_job *job, double *out, qsim_stream_t s);
+int qsim_hip_split_destroy(qsim_split_job *job);
+
+/* Many same-shape split jobs advanced in one launch per sweep. */
+int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps,
+                                 qsim_stream_t s);
+
+/* ------------------------------------------------------ fused Born job
+ * QuPerturbation2DImpl/3DImpl (src/Perturbation2DImpl.cpp,
+ * src/Perturbation3DImpl.cpp) with the kernels of src/Perburbation.h. */
+typedef struct {
+    int dim;                 /* 2 or 3                                        */
+    size_t n[3];
+    double lo[3], hi[3];
+    double direction[3];     /* incident direction (normalised internally)     */
+    double mass, hbar;
+    int preconditional;      /* 0 naive BornSeries, 1 PreconditionalBornSeries */
+    int preconditioner;      /* 0 Identity, 1 Vellekoop, 2 Hao1, 3 Hao2        */
+    double slow;             /* relaxation (reference "slow", default 1)       */
+    double epsilon;          /* user epsilon (raised to the minimum when preconditional) */
+    int precision;           /* 64 or 32 (FP32 iterates, FP64 residual)        */
+} qsim_born_desc;
+
+typedef struct qsim_born_job qsim_born_job;
+/* V: device complex128 physical potential (n points).  energies: host array
+ * of nE energies; all energies share V and the grid and iterate together. */
+int qsim_hip_born_create(qsim_born_job **job, const qsim_born_desc *d, const qsim_c128 *V,
+                         const double *energies, int nE);
+/* Up to max_iter iterations per energy; an energy stops once its relative
+ * update norm |dpsi_new - dpsi_old| / |dpsi_new| drops below tol (tol <= 0:
+ * run exactly max_iter, the reference's `order` semantics).  iters[nE] and
+ * residual[nE] are host outputs (may be NULL). */
+int qsim_hip_born_iterate(qsim_born_job *job, int max_iter, double tol, int *iters,
+                          double *residual, qsim_stream_t s);
+/* Copy energy e's scattered wave (complex128) to a device buffer. */
+int qsim_hip_born_get_dpsi(qsim_born_job *job, int e, qsim_c128 *out, qsim_stream_t s);
+/* Potential after the absorbing layer and the epsilon in use, energy e. */
+int qsim_hip_born_get_veff(qsim_born_job *job, int e, qsim_c128 *out, double *eps,
+                           qsim_stream_t s);
+/* Differential cross section toward (cosx, cosy, cosz) for energy e
+ * (src/ScatteringSolverImpl.cpp:168-196, :318-361); amp = the raw sum r. */
+int qsim_hip_born_xsection(qsim_born_job *job, int e, double cosx, double cosy, double cosz,
+                           double *dsigma, qsim_c128 *amp, qsim_stream_t s);
+int qsim_hip_born_destroy(qsim_born_job *job);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

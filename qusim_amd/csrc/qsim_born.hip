@@ -1,0 +1,991 @@
+// qsim_born.hip -- Born-series scattering job (QuPerturbation2DImpl/3DImpl on
+// gfx950), many energies advanced together.
+//
+// Reference iteration (src/Perburbation.h:350-423, preconditioned):
+//   tmp  = (Re V + i(Im V + eps)) dpsi + Re V psi0          (Add3)
+//   tmp  = IFFT( FFT(tmp) / (E - T_k + i eps) ) / n          (G0)
+//   dpsi = (1 - g) dpsi + g tmp,  g = slow (1 - i V/eps) ... (Vellekoop/Hao1/Hao2)
+// naive (src/Perburbation.h:283-309): dpsi = IFFT( G0 FFT((dpsi + psi0) V) ) / n.
+//
+// Fused passes for power-of-two grids, axis order (x = axis 0 outermost):
+//   inner sweep (last axis, contiguous lines): the end of iteration n
+//       (IFFT_inner, /n, preconditioned update of dpsi, residual sums) and the
+//       start of iteration n+1 (Add3, FFT_inner) in one read/write of
+//       tmp, dpsi and V;
+//   middle axis (3D only): plain FFT_y before and IFFT_y after the outer sweep;
+//   outer sweep (axis 0, strided lines): FFT_x, G0 multiply, IFFT_x.
+// psi0 = e^{i k.r} and the absorbing layer (src/Perburbation.h:150-177) are
+// separable, so both come from per-axis tables instead of grid arrays; the
+// physical V is stored once for all energies.
+//
+// Precision 64: complex128 iterates.  Precision 32: complex64 iterates with
+// FP64 V, FP64 phase/absorber tables and FP64 residual sums.
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "qsim_fft_kernels.h"
+#include "qsim_fft_plan.h"
+
+namespace qsim {
+
+enum { BA_START = 0, BA_END = 1, BA_BOTH = 2 };
+enum { PRE_IDENTITY = 0, PRE_VELLEKOOP = 1, PRE_HAO1 = 2, PRE_HAO2 = 3, PRE_NAIVE = 4 };
+
+struct BornGeom {
+    int dim;
+    size_t n0, n1, n2;   // n2 = 1 for 2D
+    size_t ninner;       // length of the last axis
+    size_t nmid;         // 3D: n1; 2D: 1
+};
+
+// per-energy device descriptor
+struct BornDev {
+    void *dpsi;
+    void *tmp;
+    const c128 *pw[3];     // plane-wave factor per axis: exp(i k_a x_a)
+    const double *ab[3];   // absorbing-layer Im V per axis (zeros when naive)
+    double E, eps, slow, inv_n;
+    double a[3];           // (2 pi hbar / (d n))^2 / 2m per axis
+    double *resid;         // per-block residual partial sums [2 * nblocks]
+};
+
+// potential with the absorbing layer, in the reference's summation order
+__device__ __forceinline__ c128 v_eff(c128 v, const BornDev &J, int dim, size_t i, size_t j,
+                                      size_t k)
+{
+    double a = dim == 3 ? (J.ab[0][i] + J.ab[1][j]) + J.ab[2][k] : J.ab[0][i] + J.ab[1][j];
+    return mk(v.x, v.y + a);
+}
+
+__device__ __forceinline__ c128 psi0_at(const BornDev &J, int dim, size_t i, size_t j, size_t k)
+{
+    return dim == 3 ? (J.pw[0][i] * J.pw[1][j]) * J.pw[2][k] : J.pw[0][i] * J.pw[1][j];
+}
+
+template <int PRE>
+__device__ __forceinline__ c128 update(c128 dpsi, c128 t, c128 v, double slow, double eps)
+{
+    if constexpr (PRE == PRE_NAIVE) {
+        return t;
+    } else if constexpr (PRE == PRE_IDENTITY) {
+        return slow == 1. ? t : dpsi * (1 - slow) + t * slow;
+    } else {
+        c128 g = born_gamma<PRE>(v, slow, eps);
+        return mk(1. - g.x, -g.y) * dpsi + g * t;
+    }
+}
+
+template <int PRE>
+__device__ __forceinline__ c128 source(c128 dpsi, c128 p0, c128 v, double eps)
+{
+    if constexpr (PRE == PRE_NAIVE) return (dpsi + p0) * v;          // op_AddMul
+    else return mk(v.x, v.y + eps) * dpsi + v.x * p0;                // op_Add3
+}
+
+// Inner sweep: lines along the last axis.  Line g -> outer indices
+// (i, j) = (g / nmid, g % nmid) in 3D, i = g in 2D.
+template <typename T, int N, int MODE, int PRE>
+__global__ void __launch_bounds__((P2Launch<T, N, false>::THREADS))
+    k_born_inner(const BornDev *__restrict__ jobs, const c128 *__restrict__ V, BornGeom geo,
+                 const cx<T> *__restrict__ tw)
+{
+    using LP = P2Launch<T, N, false>;
+    using P = typename LP::Line;
+    constexpr int L = LP::L;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const BornDev &J = jobs[blockIdx.y];
+    const int tid = threadIdx.x;
+    const int l = tid / P::TPL, t = tid % P::TPL;
+    const size_t nlines = geo.n0 * geo.nmid;
+    const size_t g = (size_t)blockIdx.x * L + l;
+    const bool active = g < nlines;
+    const size_t gg = active ? g : 0;
+    const size_t base = gg * N;
+    const size_t i = gg / geo.nmid, j = gg % geo.nmid;
+    cx<T> *tmp = reinterpret_cast<cx<T> *>(J.tmp);
+    cx<T> *dp = reinterpret_cast<cx<T> *>(J.dpsi);
+    constexpr int RA = P::RA_, RZ = P::RLAST;
+    const int dim = geo.dim;
+    // element (axis indices) of the inner position k
+    auto ijk = [&](int k, size_t &a, size_t &b, size_t &c) {
+        if (dim == 3) {
+            a = i;
+            b = j;
+            c = (size_t)k;
+        } else {
+            a = i;
+            b = (size_t)k;
+            c = 0;
+        }
+    };
+    cx<T> v[P::E];
+    double rsum = 0, nsum = 0;
+    if constexpr (MODE != BA_START) {
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                v[u * RZ + r] = active ? tmp[base + P::template in_pos<RZ>(t, u, r)] : mk<T>(0, 0);
+        P::inverse(v, t, lds + l * P::LS, tw);
+    }
+#pragma unroll
+    for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+        for (int r = 0; r < RA; ++r) {
+            const int k = P::template in_pos<RA>(t, u, r);
+            size_t a, b, c;
+            ijk(k, a, b, c);
+            const c128 vv = active ? V[base + k] : mk(0.0, 0.0);
+            const c128 ve = PRE == PRE_NAIVE ? vv : v_eff(vv, J, dim, a, b, c);
+            c128 d = active ? cvt<double>(dp[base + k]) : mk(0.0, 0.0);
+            if constexpr (MODE != BA_START) {
+                const c128 tn = cvt<double>(v[u * RA + r]) * J.inv_n;
+                const c128 nd = update<PRE>(d, tn, ve, J.slow, J.eps);
+                const cx<T> nds = cvt<T>(nd);
+                if (active) dp[base + k] = nds;
+                const c128 ndr = cvt<double>(nds);
+                rsum += abs2(ndr - d);
+                nsum += abs2(ndr);
+                d = ndr;
+            }
+            if constexpr (MODE != BA_END) {
+                const c128 p0 = psi0_at(J, dim, a, b, c);
+                v[u * RA + r] = cvt<T>(source<PRE>(d, p0, ve, J.eps));
+            }
+        }
+    if constexpr (MODE != BA_END) {
+        P::forward(v, t, lds + l * P::LS, tw);
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+                for (int r = 0; r < RZ; ++r)
+                    tmp[base + P::template in_pos<RZ>(t, u, r)] = v[u * RZ + r];
+        }
+    }
+    if constexpr (MODE != BA_START) {
+        // block sums of the update residual and the new norm
+        __shared__ double red[2][1024];
+        __syncthreads();
+        red[0][tid] = rsum;
+        red[1][tid] = nsum;
+        __syncthreads();
+        for (int w = LP::THREADS / 2; w > 0; w >>= 1) {
+            if (tid < w) {
+                red[0][tid] += red[0][tid + w];
+                red[1][tid] += red[1][tid + w];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            J.resid[2 * blockIdx.x] = red[0][0];
+            J.resid[2 * blockIdx.x + 1] = red[1][0];
+        }
+    }
+}
+
+// Outer sweep: lines along axis 0, stride S = n1*n2, G0 between FFT and IFFT.
+template <typename T, int N>
+__global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
+    k_born_outer(const BornDev *__restrict__ jobs, BornGeom geo, const cx<T> *__restrict__ tw)
+{
+    using LP = P2Launch<T, N, true>;
+    using P = typename LP::Line;
+    constexpr int L = LP::L;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const BornDev &J = jobs[blockIdx.y];
+    const int tid = threadIdx.x;
+    const int l = tid % L, t = tid / L;
+    const size_t S = geo.n1 * geo.n2;
+    const size_t cidx = (size_t)blockIdx.x * L + l;
+    const bool active = cidx < S;
+    const size_t c = active ? cidx : 0;
+    cx<T> *tmp = reinterpret_cast<cx<T> *>(J.tmp);
+    // fold indices of the other axes (op_G02D / op_G03D index order)
+    const double fj = geo.dim == 3 ? (double)fold_half(c / geo.n2, geo.n1)
+                                   : (double)fold_half(c, geo.n1);
+    const double fk = geo.dim == 3 ? (double)fold_half(c % geo.n2, geo.n2) : 0.0;
+    constexpr int RA = P::RA_, RZ = P::RLAST;
+    cx<T> v[P::E];
+#pragma unroll
+    for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+        for (int r = 0; r < RA; ++r)
+            v[u * RA + r] = active ? tmp[c + (size_t)P::template in_pos<RA>(t, u, r) * S]
+                                   : mk<T>(0, 0);
+    P::forward(v, t, lds + l * P::LS, tw);
+#pragma unroll
+    for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) {
+            const int kk = P::template in_pos<RZ>(t, u, r);
+            const double fi = (double)fold_half((size_t)kk, (size_t)N);
+            const double tt = geo.dim == 3
+                                  ? fi * fi * J.a[0] + fj * fj * J.a[1] + fk * fk * J.a[2]
+                                  : fi * fi * J.a[0] + fj * fj * J.a[1];
+            const c128 z = cdiv(cvt<double>(v[u * RZ + r]), mk(J.E - tt, J.eps));
+            v[u * RZ + r] = cvt<T>(z);
+        }
+    P::inverse(v, t, lds + l * P::LS, tw);
+    if (active) {
+#pragma unroll
+        for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+            for (int r = 0; r < RA; ++r)
+                tmp[c + (size_t)P::template in_pos<RA>(t, u, r) * S] = v[u * RA + r];
+    }
+}
+
+// Middle axis (3D y): plain transform of lines [n0][n1][n2] along axis 1.
+template <typename T, int N, bool INV>
+__global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
+    k_born_mid(const BornDev *__restrict__ jobs, BornGeom geo, const cx<T> *__restrict__ tw)
+{
+    using LP = P2Launch<T, N, true>;
+    using P = typename LP::Line;
+    constexpr int L = LP::L;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const BornDev &J = jobs[blockIdx.y];
+    const int tid = threadIdx.x;
+    const int l = tid % L, t = tid / L;
+    const size_t S = geo.n2;
+    const size_t nlines = geo.n0 * geo.n2;
+    const size_t g = (size_t)blockIdx.x * L + l;
+    const bool active = g < nlines;
+    const size_t base = active ? line_base(g, N, S) : 0;
+    cx<T> *tmp = reinterpret_cast<cx<T> *>(J.tmp);
+    constexpr int R0 = INV ? P::RLAST : P::RA_;
+    constexpr int R1 = INV ? P::RA_ : P::RLAST;
+    cx<T> v[P::E];
+#pragma unroll
+    for (int u = 0; u < P::E / R0; ++u)
+#pragma unroll
+        for (int r = 0; r < R0; ++r)
+            v[u * R0 + r] = active ? tmp[base + (size_t)P::template in_pos<R0>(t, u, r) * S]
+                                   : mk<T>(0, 0);
+    if constexpr (INV)
+        P::inverse(v, t, lds + l * P::LS, tw);
+    else
+        P::forward(v, t, lds + l * P::LS, tw);
+    if (active) {
+#pragma unroll
+        for (int u = 0; u < P::E / R1; ++u)
+#pragma unroll
+            for (int r = 0; r < R1; ++r)
+                tmp[base + (size_t)P::template in_pos<R1>(t, u, r) * S] = v[u * R1 + r];
+    }
+}
+
+template <typename F>
+__global__ void __launch_bounds__(256) k_each_b(size_t n, F f)
+{
+    size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) f(i);
+}
+
+template <typename F>
+int each_b(size_t n, hipStream_t s, F f)
+{
+    if (!n) return QSIM_OK;
+    size_t b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_each_b<F>, dim3((unsigned)b), dim3(256), 0, s, n, f);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+using cd = std::complex<double>;
+
+}  // namespace qsim
+
+using namespace qsim;
+
+struct qsim_born_job {
+    qsim_born_desc d{};
+    BornGeom geo{};
+    size_t n = 0;
+    int nE = 0;
+    int pre = PRE_NAIVE;
+    bool fused = false;
+    int prec = 64;
+    const c128 *V = nullptr;    // caller's physical potential (device)
+    std::vector<double> E;
+    std::vector<BornDev> host;  // per energy
+    BornDev *dev = nullptr;     // device copy of `host`
+    BornDev *active_tab = nullptr;
+    std::vector<void *> owned;
+    std::vector<bool> started;  // tmp holds a started iteration
+    std::vector<int> iters;
+    std::vector<double> resid;
+    size_t inner_blocks = 0;
+    const void *tw[3] = {nullptr, nullptr, nullptr};
+    // unfused path (FP64, any size)
+    qsim_fft_plan *fwd = nullptr, *inv = nullptr;
+    c128 *psi0 = nullptr, *veff = nullptr, *last = nullptr;
+};
+
+namespace {
+
+int balloc(qsim_born_job *j, void **p, size_t bytes)
+{
+    QSIM_HIP(hipMalloc(p, bytes));
+    j->owned.push_back(*p);
+    return QSIM_OK;
+}
+
+template <typename T>
+int upload(qsim_born_job *j, const std::vector<T> &h, const T **out)
+{
+    void *p;
+    QSIM_TRY(balloc(j, &p, h.size() * sizeof(T)));
+    QSIM_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T *)p;
+    return QSIM_OK;
+}
+
+// cal_v15 (src/Perburbation.h:45-51)
+double cal_v15(double x, double k0, double e0)
+{
+    x *= k0;
+    return (-0.8 * e0) *
+           ((1 - x * x / (15. * 15)) > 0 ? std::exp(-0.015 * x * x * 1. / std::sqrt(1 - x * x / (15. * 15)))
+                                         : 0);
+}
+
+size_t pfold(size_t i, size_t n) { return (i < n / 2) ? i : n - i; }
+
+bool born_len_ok(size_t n, int prec)
+{
+    if (n < 64 || (n & (n - 1))) return false;
+    return prec == 64 ? n <= 4096 : n <= 8192;
+}
+
+#define QSIM_BSWITCH(PREC, NVAL, CALL)                                                \
+    switch (NVAL) {                                                                   \
+        case 64: return PREC == 64 ? CALL(double, 64) : CALL(float, 64);              \
+        case 128: return PREC == 64 ? CALL(double, 128) : CALL(float, 128);           \
+        case 256: return PREC == 64 ? CALL(double, 256) : CALL(float, 256);           \
+        case 512: return PREC == 64 ? CALL(double, 512) : CALL(float, 512);           \
+        case 1024: return PREC == 64 ? CALL(double, 1024) : CALL(float, 1024);        \
+        case 2048: return PREC == 64 ? CALL(double, 2048) : CALL(float, 2048);        \
+        case 4096: return PREC == 64 ? CALL(double, 4096) : CALL(float, 4096);        \
+        default: break;                                                               \
+    }                                                                                 \
+    return set_error(QSIM_ERR_UNSUPPORTED, "no fused Born kernel for this length");
+
+template <typename T, int N, int MODE, int PRE>
+int launch_inner_t(const BornDev *tab, int nact, const c128 *V, const BornGeom &geo,
+                   const void *tw, hipStream_t s)
+{
+    using LP = P2Launch<T, N, false>;
+    auto fn = k_born_inner<T, N, MODE, PRE>;
+    if (LP::LDS_BYTES > 65536)
+        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)LP::LDS_BYTES));
+    size_t nlines = geo.n0 * geo.nmid;
+    dim3 grid((unsigned)((nlines + LP::L - 1) / LP::L), (unsigned)nact);
+    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, V, geo,
+                       (const cx<T> *)tw);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+template <typename T, int N>
+size_t inner_blocks_t(const BornGeom &geo)
+{
+    using LP = P2Launch<T, N, false>;
+    return (geo.n0 * geo.nmid + LP::L - 1) / LP::L;
+}
+
+template <typename T, int N, int MODE>
+int launch_inner_m(int pre, const BornDev *tab, int nact, const c128 *V, const BornGeom &geo,
+                   const void *tw, hipStream_t s)
+{
+    switch (pre) {
+        case PRE_IDENTITY: return launch_inner_t<T, N, MODE, PRE_IDENTITY>(tab, nact, V, geo, tw, s);
+        case PRE_VELLEKOOP: return launch_inner_t<T, N, MODE, PRE_VELLEKOOP>(tab, nact, V, geo, tw, s);
+        case PRE_HAO1: return launch_inner_t<T, N, MODE, PRE_HAO1>(tab, nact, V, geo, tw, s);
+        case PRE_HAO2: return launch_inner_t<T, N, MODE, PRE_HAO2>(tab, nact, V, geo, tw, s);
+        default: return launch_inner_t<T, N, MODE, PRE_NAIVE>(tab, nact, V, geo, tw, s);
+    }
+}
+
+template <typename T, int N>
+int launch_inner_n(int mode, int pre, const BornDev *tab, int nact, const c128 *V,
+                   const BornGeom &geo, const void *tw, hipStream_t s)
+{
+    if (mode == BA_START) return launch_inner_m<T, N, BA_START>(pre, tab, nact, V, geo, tw, s);
+    if (mode == BA_END) return launch_inner_m<T, N, BA_END>(pre, tab, nact, V, geo, tw, s);
+    return launch_inner_m<T, N, BA_BOTH>(pre, tab, nact, V, geo, tw, s);
+}
+
+int inner(int prec, int mode, int pre, const BornDev *tab, int nact, const c128 *V,
+          const BornGeom &geo, const void *tw, hipStream_t s)
+{
+#define INNERCALL(T, NN) launch_inner_n<T, NN>(mode, pre, tab, nact, V, geo, tw, s)
+    QSIM_BSWITCH(prec, geo.ninner, INNERCALL)
+#undef INNERCALL
+}
+
+size_t inner_blocks(int prec, const BornGeom &geo)
+{
+#define IB(NN) \
+    case NN: return prec == 64 ? inner_blocks_t<double, NN>(geo) : inner_blocks_t<float, NN>(geo);
+    switch (geo.ninner) {
+        IB(64) IB(128) IB(256) IB(512) IB(1024) IB(2048) IB(4096)
+        default: return 0;
+    }
+#undef IB
+}
+
+template <typename T, int N>
+int launch_outer_t(const BornDev *tab, int nact, const BornGeom &geo, const void *tw,
+                   hipStream_t s)
+{
+    using LP = P2Launch<T, N, true>;
+    auto fn = k_born_outer<T, N>;
+    if (LP::LDS_BYTES > 65536)
+        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)LP::LDS_BYTES));
+    size_t S = geo.n1 * geo.n2;
+    dim3 grid((unsigned)((S + LP::L - 1) / LP::L), (unsigned)nact);
+    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, geo, (const cx<T> *)tw);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+int outer(int prec, const BornDev *tab, int nact, const BornGeom &geo, const void *tw,
+          hipStream_t s)
+{
+#define OUTERCALL(T, NN) launch_outer_t<T, NN>(tab, nact, geo, tw, s)
+    QSIM_BSWITCH(prec, geo.n0, OUTERCALL)
+#undef OUTERCALL
+}
+
+template <typename T, int N, bool INV>
+int launch_mid_t(const BornDev *tab, int nact, const BornGeom &geo, const void *tw,
+                 hipStream_t s)
+{
+    using LP = P2Launch<T, N, true>;
+    auto fn = k_born_mid<T, N, INV>;
+    if (LP::LDS_BYTES > 65536)
+        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)LP::LDS_BYTES));
+    size_t nlines = geo.n0 * geo.n2;
+    dim3 grid((unsigned)((nlines + LP::L - 1) / LP::L), (unsigned)nact);
+    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, geo, (const cx<T> *)tw);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+int mid(int prec, bool inv, const BornDev *tab, int nact, const BornGeom &geo, const void *tw,
+        hipStream_t s)
+{
+#define MIDCALL(T, NN) (inv ? launch_mid_t<T, NN, true>(tab, nact, geo, tw, s) \
+                            : launch_mid_t<T, NN, false>(tab, nact, geo, tw, s))
+    QSIM_BSWITCH(prec, geo.n1, MIDCALL)
+#undef MIDCALL
+}
+
+void destroy(qsim_born_job *j)
+{
+    if (!j) return;
+    for (void *p : j->owned) (void)hipFree(p);
+    if (j->fwd) qsim_hip_fft_plan_destroy(j->fwd);
+    if (j->inv) qsim_hip_fft_plan_destroy(j->inv);
+    delete j;
+}
+
+// epsilon rule (src/Perburbation.h:329-344) over V_eff of energy e
+int epsilon_for(qsim_born_job *j, int e, double *eps)
+{
+    const BornDev J = j->host[e];
+    const BornGeom geo = j->geo;
+    const c128 *V = j->V;
+    const size_t n = j->n;
+    void *p;
+    QSIM_HIP(hipMalloc(&p, n * sizeof(c128)));
+    c128 *ve = (c128 *)p;
+    int rc = each_b(n, nullptr, [=] __device__(size_t idx) {
+        size_t i, jj, k;
+        if (geo.dim == 3) {
+            k = idx % geo.n2;
+            jj = (idx / geo.n2) % geo.n1;
+            i = idx / (geo.n1 * geo.n2);
+        } else {
+            k = 0;
+            jj = idx % geo.n1;
+            i = idx / geo.n1;
+        }
+        ve[idx] = v_eff(V[idx], J, geo.dim, i, jj, k);
+    });
+    double minEps = 0;
+    if (rc == QSIM_OK) {
+        if (j->pre == PRE_IDENTITY) {
+            double A = 0, A2 = 0;
+            rc = qsim_hip_sum_imag((const qsim_c128 *)ve, n, &A, nullptr);
+            if (rc == QSIM_OK) rc = qsim_hip_norm2((const qsim_c128 *)ve, n, &A2, nullptr);
+            minEps = 0.5 * A2 / (-A);
+        } else {
+            rc = qsim_hip_max_abs((const qsim_c128 *)ve, n, &minEps, nullptr);
+        }
+    }
+    (void)hipFree(p);
+    if (rc != QSIM_OK) return rc;
+    if (*eps < minEps) *eps = minEps;
+    return QSIM_OK;
+}
+
+int create(qsim_born_job *j)
+{
+    const qsim_born_desc &d = j->d;
+    const int dim = d.dim;
+    BornGeom &g = j->geo;
+    g.dim = dim;
+    g.n0 = d.n[0];
+    g.n1 = d.n[1];
+    g.n2 = dim == 3 ? d.n[2] : 1;
+    g.ninner = dim == 3 ? g.n2 : g.n1;
+    g.nmid = dim == 3 ? g.n1 : 1;
+    j->n = g.n0 * g.n1 * g.n2;
+    j->pre = d.preconditional ? d.preconditioner : PRE_NAIVE;
+    j->prec = d.precision;
+    j->fused = born_len_ok(g.n0, j->prec) && born_len_ok(g.ninner, j->prec) &&
+               (dim == 2 || born_len_ok(g.n1, j->prec));
+    if (!j->fused && j->prec != 64)
+        return set_error(QSIM_ERR_UNSUPPORTED,
+                         "FP32 Born job needs power-of-two axes (64..4096)");
+    double dx[3] = {0, 0, 0};
+    for (int a = 0; a < dim; ++a) dx[a] = (d.hi[a] - d.lo[a]) / d.n[a];
+    double nm2 = 0;
+    for (int a = 0; a < dim; ++a) nm2 += d.direction[a] * d.direction[a];
+    QSIM_CHECK(nm2 > 0, "zero incident direction");
+    const double nm = 1 / std::sqrt(nm2);
+    const size_t esz = j->prec == 64 ? sizeof(c128) : sizeof(c64);
+    j->host.resize(j->nE);
+    j->started.assign(j->nE, false);
+    j->iters.assign(j->nE, 0);
+    j->resid.assign(j->nE, 0.0);
+    j->inner_blocks = j->fused ? inner_blocks(j->prec, g) : 0;
+    for (int e = 0; e < j->nE; ++e) {
+        BornDev &J = j->host[e];
+        const double En = j->E[e];
+        QSIM_CHECK(En > 0, "energy must be positive");
+        // k0 = sqrt(2 m E) / hbar, src/Perturbation3DImpl.cpp:14 + ScatteringSolverImpl.cpp:10
+        const double k0 = std::sqrt(2 * En * d.mass) / d.hbar;
+        if (d.preconditional) {
+            if (dim == 3) {
+                const double lambda = 20 / (std::sqrt(2 * d.mass * En) / d.hbar);
+                for (int a = 0; a < 3; ++a)
+                    if (d.n[a] * dx[a] < lambda)
+                        return set_error(QSIM_ERR_INVALID,
+                                         "too small size: lambda: " + std::to_string(lambda));
+            } else {
+                const double lambda = 2 * M_PI / (std::sqrt(2 * d.mass * En) / d.hbar);
+                for (int a = 0; a < 2; ++a)
+                    if (d.n[a] * dx[a] < 24 * lambda)
+                        return set_error(QSIM_ERR_INVALID, "too small size");
+            }
+        }
+        for (int a = 0; a < dim; ++a) {
+            const double ka = k0 * (d.direction[a] * nm);
+            std::vector<c128> pw(d.n[a]);
+            std::vector<double> ab(d.n[a], 0.0);
+            for (size_t i = 0; i < d.n[a]; ++i) {
+                const double x = d.lo[a] + dx[a] * i;
+                const double ph = ka * x;
+                pw[i] = mk(std::cos(ph), std::sin(ph));
+                if (d.preconditional) ab[i] = cal_v15(pfold(i, d.n[a]) * dx[a], k0, En);
+            }
+            QSIM_TRY(upload(j, pw, &J.pw[a]));
+            QSIM_TRY(upload(j, ab, &J.ab[a]));
+            // CalDp2Div2M, src/Perburbation.h:16-22
+            const double Dp = 2 * M_PI / (dx[a] * d.n[a]) * d.hbar;
+            J.a[a] = Dp * Dp / (2 * d.mass);
+        }
+        if (dim == 2) {
+            J.pw[2] = J.pw[1];
+            J.ab[2] = J.ab[1];
+            J.a[2] = 0;
+        }
+        J.E = En;
+        J.slow = d.slow;
+        J.inv_n = 1. / (double)j->n;
+        double eps = d.epsilon;
+        if (d.preconditional) QSIM_TRY(epsilon_for(j, e, &eps));
+        J.eps = eps;
+        void *p;
+        QSIM_TRY(balloc(j, &p, j->n * esz));
+        J.dpsi = p;
+        QSIM_HIP(hipMemset(p, 0, j->n * esz));
+        QSIM_TRY(balloc(j, &p, j->n * esz));
+        J.tmp = p;
+        size_t nb = j->fused ? j->inner_blocks : 1;
+        QSIM_TRY(balloc(j, &p, 2 * nb * sizeof(double)));
+        J.resid = (double *)p;
+    }
+    void *p;
+    QSIM_TRY(balloc(j, &p, sizeof(BornDev) * j->nE));
+    j->dev = (BornDev *)p;
+    QSIM_HIP(hipMemcpy(j->dev, j->host.data(), sizeof(BornDev) * j->nE, hipMemcpyHostToDevice));
+    QSIM_TRY(balloc(j, &p, sizeof(BornDev) * j->nE));
+    j->active_tab = (BornDev *)p;
+    if (j->fused) {
+        QSIM_TRY(twiddle_table(g.n0, j->prec, &j->tw[0]));
+        QSIM_TRY(twiddle_table(g.n1, j->prec, &j->tw[1]));
+        if (dim == 3) QSIM_TRY(twiddle_table(g.n2, j->prec, &j->tw[2]));
+    } else {
+        size_t dims[3] = {d.n[0], d.n[1], d.n[2]};
+        QSIM_TRY(qsim_hip_fft_plan_create(&j->fwd, dim, dims, 0, 64));
+        QSIM_TRY(qsim_hip_fft_plan_create(&j->inv, dim, dims, 1, 64));
+        QSIM_TRY(balloc(j, &p, j->n * sizeof(c128)));
+        j->psi0 = (c128 *)p;
+        QSIM_TRY(balloc(j, &p, j->n * sizeof(c128)));
+        j->veff = (c128 *)p;
+        QSIM_TRY(balloc(j, &p, j->n * sizeof(c128)));
+        j->last = (c128 *)p;
+    }
+    QSIM_HIP(hipDeviceSynchronize());
+    return QSIM_OK;
+}
+
+int fused_pass_set(qsim_born_job *j, int mode, const BornDev *tab, int nact, hipStream_t s)
+{
+    // mode: BA_START or BA_BOTH starts an iteration and runs it up to the
+    // point where only the inner inverse + update remains
+    const void *tw_in = j->geo.dim == 3 ? j->tw[2] : j->tw[1];
+    QSIM_TRY(inner(j->prec, mode, j->pre, tab, nact, j->V, j->geo, tw_in, s));
+    if (j->geo.dim == 3) QSIM_TRY(mid(j->prec, false, tab, nact, j->geo, j->tw[1], s));
+    QSIM_TRY(outer(j->prec, tab, nact, j->geo, j->tw[0], s));
+    if (j->geo.dim == 3) QSIM_TRY(mid(j->prec, true, tab, nact, j->geo, j->tw[1], s));
+    return QSIM_OK;
+}
+
+int read_residuals(qsim_born_job *j, const std::vector<int> &act, hipStream_t s,
+                   std::vector<double> &out)
+{
+    const size_t nb = j->inner_blocks;
+    std::vector<double> h(2 * nb);
+    out.assign(act.size(), 0.0);
+    QSIM_HIP(hipStreamSynchronize(s));
+    for (size_t a = 0; a < act.size(); ++a) {
+        QSIM_HIP(hipMemcpy(h.data(), j->host[act[a]].resid, 2 * nb * sizeof(double),
+                           hipMemcpyDeviceToHost));
+        double r = 0, nn = 0;
+        for (size_t b = 0; b < nb; ++b) {
+            r += h[2 * b];
+            nn += h[2 * b + 1];
+        }
+        out[a] = nn > 0 ? std::sqrt(r / nn) : 0.0;
+    }
+    return QSIM_OK;
+}
+
+int fused_iterate(qsim_born_job *j, int max_iter, double tol, hipStream_t s)
+{
+    std::vector<int> act;
+    for (int e = 0; e < j->nE; ++e) act.push_back(e);
+    std::vector<int> done_it(j->nE, 0);
+    const int check_every = tol > 0 ? 4 : 0;
+    int it = 0;
+    auto upload_tab = [&]() -> int {
+        std::vector<BornDev> h;
+        for (int e : act) h.push_back(j->host[e]);
+        QSIM_HIP(hipMemcpyAsync(j->active_tab, h.data(), sizeof(BornDev) * h.size(),
+                                hipMemcpyHostToDevice, s));
+        QSIM_HIP(hipStreamSynchronize(s));
+        return QSIM_OK;
+    };
+    QSIM_TRY(upload_tab());
+    // every energy of one job starts and stops on the same iteration count
+    // unless a tolerance removes it from the active table
+    bool started = j->started[0];
+    while (it < max_iter && !act.empty()) {
+        const int nact = (int)act.size();
+        QSIM_TRY(fused_pass_set(j, started ? BA_BOTH : BA_START, j->active_tab, nact, s));
+        started = true;
+        ++it;
+        for (int e : act) {
+            j->iters[e] += 1;
+            done_it[e] += 1;
+        }
+        if (check_every && it > 1 && (it % check_every) == 0) {
+            // the residuals written by the BOTH pass belong to the previous update
+            std::vector<double> r;
+            QSIM_TRY(read_residuals(j, act, s, r));
+            std::vector<int> keep;
+            for (size_t a = 0; a < act.size(); ++a) {
+                j->resid[act[a]] = r[a];
+                if (r[a] >= tol) keep.push_back(act[a]);
+                else {
+                    // converged: its started iteration is discarded (dpsi is final)
+                    j->started[act[a]] = false;
+                    j->iters[act[a]] -= 1;
+                }
+            }
+            if (keep.size() != act.size()) {
+                act = keep;
+                if (!act.empty()) QSIM_TRY(upload_tab());
+            }
+        }
+    }
+    if (!act.empty()) {
+        // finish the started iteration of every still-active energy
+        QSIM_TRY(inner(j->prec, BA_END, j->pre, j->active_tab, (int)act.size(), j->V, j->geo,
+                       j->geo.dim == 3 ? j->tw[2] : j->tw[1], s));
+        std::vector<double> r;
+        QSIM_TRY(read_residuals(j, act, s, r));
+        for (size_t a = 0; a < act.size(); ++a) {
+            j->resid[act[a]] = r[a];
+            j->started[act[a]] = false;
+        }
+    }
+    for (int e = 0; e < j->nE; ++e) j->started[e] = false;
+    return QSIM_OK;
+}
+
+// Unfused FP64 reference-order iteration for any grid the FFT plans accept.
+int unfused_iterate(qsim_born_job *j, int max_iter, double tol, hipStream_t s)
+{
+    const size_t n = j->n;
+    const BornGeom geo = j->geo;
+    const c128 *V = j->V;
+    for (int e = 0; e < j->nE; ++e) {
+        const BornDev J = j->host[e];
+        c128 *dp = (c128 *)J.dpsi, *tmp = (c128 *)J.tmp, *p0 = j->psi0, *ve = j->veff;
+        const int pre = j->pre;
+        QSIM_TRY(each_b(n, s, [=] __device__(size_t idx) {
+            size_t i, jj, k;
+            if (geo.dim == 3) {
+                k = idx % geo.n2;
+                jj = (idx / geo.n2) % geo.n1;
+                i = idx / (geo.n1 * geo.n2);
+            } else {
+                k = 0;
+                jj = idx % geo.n1;
+                i = idx / geo.n1;
+            }
+            p0[idx] = psi0_at(J, geo.dim, i, jj, k);
+            ve[idx] = pre == PRE_NAIVE ? V[idx] : v_eff(V[idx], J, geo.dim, i, jj, k);
+        }));
+        for (int it = 0; it < max_iter; ++it) {
+            QSIM_HIP(hipMemcpyAsync(j->last, dp, n * sizeof(c128), hipMemcpyDeviceToDevice, s));
+            const double eps = J.eps;
+            if (pre == PRE_NAIVE)
+                QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { tmp[i] = (dp[i] + p0[i]) * ve[i]; }));
+            else
+                QSIM_TRY(each_b(n, s, [=] __device__(size_t i) {
+                    c128 v = ve[i];
+                    tmp[i] = mk(v.x, v.y + eps) * dp[i] + v.x * p0[i];
+                }));
+            QSIM_TRY(qsim_hip_fft_exec(j->fwd, tmp, tmp, s));
+            if (geo.dim == 3)
+                QSIM_TRY(qsim_hip_g03d((qsim_c128 *)tmp, J.E, eps, J.a[0], J.a[1], J.a[2], geo.n0,
+                                       geo.n1, geo.n2, s));
+            else
+                QSIM_TRY(qsim_hip_g02d((qsim_c128 *)tmp, J.E, eps, J.a[0], J.a[1], geo.n0, geo.n1, s));
+            QSIM_TRY(qsim_hip_fft_exec(j->inv, tmp, tmp, s));
+            const double in = J.inv_n, slow = J.slow;
+            switch (pre) {
+                case PRE_NAIVE:
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_NAIVE>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    break;
+                case PRE_IDENTITY:
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_IDENTITY>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    break;
+                case PRE_VELLEKOOP:
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_VELLEKOOP>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    break;
+                case PRE_HAO1:
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_HAO1>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    break;
+                default:
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_HAO2>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    break;
+            }
+            j->iters[e] += 1;
+            double num = 0, den = 0;
+            QSIM_TRY(qsim_hip_minus_norm2((qsim_c128 *)dp, (qsim_c128 *)j->last, n, &num, s));
+            QSIM_TRY(qsim_hip_norm2((qsim_c128 *)dp, n, &den, s));
+            j->resid[e] = den > 0 ? std::sqrt(num / den) : 0.0;
+            if (tol > 0 && j->resid[e] < tol) break;
+        }
+    }
+    return QSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsim_hip_born_create(qsim_born_job **job, const qsim_born_desc *d, const qsim_c128 *V,
+                         const double *energies, int nE)
+{
+    QSIM_CHECK(job && d && V && energies, "null argument");
+    *job = nullptr;
+    QSIM_CHECK(d->dim == 2 || d->dim == 3, "dim must be 2 or 3");
+    QSIM_CHECK(nE >= 1, "need at least one energy");
+    QSIM_CHECK(d->precision == 64 || d->precision == 32, "precision must be 64 or 32");
+    QSIM_CHECK(!d->preconditional || (d->preconditioner >= 0 && d->preconditioner <= 3),
+               "Unknown preconditioner");
+    QSIM_CHECK(d->mass != 0 && d->hbar != 0, "mass and hbar must be non-zero");
+    for (int a = 0; a < d->dim; ++a) {
+        QSIM_CHECK(d->n[a] >= 1, "empty grid");
+        QSIM_CHECK(d->hi[a] != d->lo[a], "empty domain");
+    }
+    qsim_born_job *j = new qsim_born_job();
+    j->d = *d;
+    if (d->dim == 2) j->d.n[2] = 1;
+    j->V = reinterpret_cast<const c128 *>(V);
+    j->nE = nE;
+    j->E.assign(energies, energies + nE);
+    int rc = create(j);
+    if (rc != QSIM_OK) {
+        destroy(j);
+        return rc;
+    }
+    *job = j;
+    return QSIM_OK;
+}
+
+int qsim_hip_born_iterate(qsim_born_job *j, int max_iter, double tol, int *iters,
+                          double *residual, qsim_stream_t stream)
+{
+    QSIM_CHECK(j, "null job");
+    QSIM_CHECK(max_iter >= 0, "max_iter must be >= 0");
+    hipStream_t s = as_stream(stream);
+    for (int e = 0; e < j->nE; ++e) j->iters[e] = 0;
+    if (max_iter > 0) {
+        if (j->fused) QSIM_TRY(fused_iterate(j, max_iter, tol, s));
+        else QSIM_TRY(unfused_iterate(j, max_iter, tol, s));
+    }
+    QSIM_HIP(hipStreamSynchronize(s));
+    for (int e = 0; e < j->nE; ++e) {
+        if (iters) iters[e] = j->iters[e];
+        if (residual) residual[e] = j->resid[e];
+    }
+    return QSIM_OK;
+}
+
+int qsim_hip_born_get_dpsi(qsim_born_job *j, int e, qsim_c128 *out, qsim_stream_t stream)
+{
+    QSIM_CHECK(j && out, "null argument");
+    QSIM_CHECK(e >= 0 && e < j->nE, "energy index out of range");
+    hipStream_t s = as_stream(stream);
+    c128 *o = reinterpret_cast<c128 *>(out);
+    if (j->prec == 64) {
+        QSIM_HIP(hipMemcpyAsync(o, j->host[e].dpsi, j->n * sizeof(c128), hipMemcpyDeviceToDevice, s));
+    } else {
+        const c64 *src = (const c64 *)j->host[e].dpsi;
+        QSIM_TRY(each_b(j->n, s, [=] __device__(size_t i) { o[i] = cvt<double>(src[i]); }));
+    }
+    return QSIM_OK;
+}
+
+int qsim_hip_born_get_veff(qsim_born_job *j, int e, qsim_c128 *out, double *eps,
+                           qsim_stream_t stream)
+{
+    QSIM_CHECK(j && out, "null argument");
+    QSIM_CHECK(e >= 0 && e < j->nE, "energy index out of range");
+    const BornDev J = j->host[e];
+    const BornGeom geo = j->geo;
+    const c128 *V = j->V;
+    const int pre = j->pre;
+    c128 *o = reinterpret_cast<c128 *>(out);
+    QSIM_TRY(each_b(j->n, as_stream(stream), [=] __device__(size_t idx) {
+        size_t i, jj, k;
+        if (geo.dim == 3) {
+            k = idx % geo.n2;
+            jj = (idx / geo.n2) % geo.n1;
+            i = idx / (geo.n1 * geo.n2);
+        } else {
+            k = 0;
+            jj = idx % geo.n1;
+            i = idx / geo.n1;
+        }
+        o[idx] = pre == PRE_NAIVE ? V[idx] : v_eff(V[idx], J, geo.dim, i, jj, k);
+    }));
+    if (eps) *eps = J.eps;
+    return QSIM_OK;
+}
+
+int qsim_hip_born_xsection(qsim_born_job *j, int e, double cosx, double cosy, double cosz,
+                           double *dsigma, qsim_c128 *amp, qsim_stream_t stream)
+{
+    QSIM_CHECK(j, "null job");
+    QSIM_CHECK(e >= 0 && e < j->nE, "energy index out of range");
+    const qsim_born_desc &d = j->d;
+    const BornGeom geo = j->geo;
+    const double En = j->E[e];
+    const double k0 = std::sqrt(2 * En * d.mass) / d.hbar;
+    double dx[3] = {1, 1, 1};
+    for (int a = 0; a < d.dim; ++a) dx[a] = (d.hi[a] - d.lo[a]) / d.n[a];
+    // psi0 + dpsi as complex128 in a scratch array, then the device reduction
+    void *p;
+    QSIM_HIP(hipMalloc(&p, j->n * sizeof(c128)));
+    c128 *tot = (c128 *)p;
+    const BornDev J = j->host[e];
+    const int prec = j->prec;
+    hipStream_t s = as_stream(stream);
+    int rc = each_b(j->n, s, [=] __device__(size_t idx) {
+        size_t i, jj, k;
+        if (geo.dim == 3) {
+            k = idx % geo.n2;
+            jj = (idx / geo.n2) % geo.n1;
+            i = idx / (geo.n1 * geo.n2);
+        } else {
+            k = 0;
+            jj = idx % geo.n1;
+            i = idx / geo.n1;
+        }
+        c128 dp = prec == 64 ? ((const c128 *)J.dpsi)[idx] : cvt<double>(((const c64 *)J.dpsi)[idx]);
+        tot[idx] = psi0_at(J, geo.dim, i, jj, k) + dp;
+    });
+    qsim_c128 r{0, 0};
+    // a zero psi0 argument: tot already holds psi0 + dpsi
+    void *zp = nullptr;
+    if (rc == QSIM_OK) rc = hipMalloc(&zp, j->n * sizeof(c128)) == hipSuccess ? QSIM_OK : QSIM_ERR_NOMEM;
+    if (rc == QSIM_OK) rc = hipMemsetAsync(zp, 0, j->n * sizeof(c128), s) == hipSuccess ? QSIM_OK : QSIM_ERR_HIP;
+    if (rc == QSIM_OK) {
+        if (d.dim == 3) {
+            double nm = 1 / std::sqrt(cosx * cosx + cosy * cosy + cosz * cosz);
+            rc = qsim_hip_xsection3d((const qsim_c128 *)zp, (const qsim_c128 *)tot,
+                                     (const qsim_c128 *)j->V, k0 * cosx * nm, k0 * cosy * nm,
+                                     k0 * cosz * nm, d.lo[0], d.lo[1], d.lo[2], dx[0], dx[1], dx[2],
+                                     d.n[0], d.n[1], d.n[2], &r, stream);
+        } else {
+            double nm = 1 / std::sqrt(cosx * cosx + cosy * cosy);
+            rc = qsim_hip_xsection2d((const qsim_c128 *)zp, (const qsim_c128 *)tot,
+                                     (const qsim_c128 *)j->V, k0 * cosx * nm, k0 * cosy * nm,
+                                     d.lo[0], d.lo[1], dx[0], dx[1], d.n[0], d.n[1], &r, stream);
+        }
+    }
+    (void)hipFree(p);
+    if (zp) (void)hipFree(zp);
+    if (rc != QSIM_OK) return rc;
+    const cd rr(r.re, r.im);
+    double ds;
+    if (d.dim == 3) {
+        // src/ScatteringSolverImpl.cpp:355-357
+        ds = std::norm(rr * (2 * d.mass) / (d.hbar * d.hbar) * dx[0] * dx[1] * dx[2] / (4 * M_PI));
+    } else {
+        // src/ScatteringSolverImpl.cpp:193
+        ds = std::norm(rr * (2 * d.mass) / (d.hbar * d.hbar) * dx[0] * dx[1] * (1. / 4)) *
+             (2 / (M_PI * k0));
+    }
+    if (dsigma) *dsigma = ds;
+    if (amp) *amp = r;
+    return QSIM_OK;
+}
+
+int qsim_hip_born_destroy(qsim_born_job *j)
+{
+    destroy(j);
+    return QSIM_OK;
+}
+
+}  // extern "C"

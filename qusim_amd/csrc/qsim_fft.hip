@@ -1,0 +1,272 @@
+// qsim_fft.hip -- FourierTransform{1,2,3}D plans on gfx950 (replaces the
+// cuFFT Z2Z plans of src/FourierTransform.cpp:84-128, 336-360, 495-519 and
+// the KISS/FFTW CPU transforms).  Power-of-two axes up to 4096 (FP64) / 8192
+// (FP32) run the register-resident Stockham kernel; every other length that
+// fits twice in LDS runs the generic mixed-radix kernel.
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "qsim_fft_kernels.h"
+#include "qsim_fft_plan.h"
+
+namespace qsim {
+
+// ------------------------------------------------------------ twiddle tables
+// tw[m] = exp(-2 pi i m / n), m < n, from x87 long double, rounded once to
+// double and (for FP32) from the double to float.
+static std::mutex g_tw_mu;
+static std::map<std::pair<int, std::pair<size_t, int>>, void *> g_tw;
+
+int twiddle_table(size_t n, int precision, const void **out)
+{
+    int dev = 0;
+    QSIM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto key = std::make_pair(dev, std::make_pair(n, precision));
+    auto it = g_tw.find(key);
+    if (it != g_tw.end()) {
+        *out = it->second;
+        return QSIM_OK;
+    }
+    std::vector<double> h(2 * n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (size_t m = 0; m < n; ++m) {
+        long double a = -2.0L * pi * (long double)m / (long double)n;
+        h[2 * m] = (double)cosl(a);
+        h[2 * m + 1] = (double)sinl(a);
+    }
+    void *d = nullptr;
+    if (precision == 64) {
+        QSIM_HIP(hipMalloc(&d, n * sizeof(c128)));
+        QSIM_HIP(hipMemcpy(d, h.data(), n * sizeof(c128), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(2 * n);
+        for (size_t i = 0; i < 2 * n; ++i) f[i] = (float)h[i];
+        QSIM_HIP(hipMalloc(&d, n * sizeof(c64)));
+        QSIM_HIP(hipMemcpy(d, f.data(), n * sizeof(c64), hipMemcpyHostToDevice));
+    }
+    g_tw[key] = d;
+    *out = d;
+    return QSIM_OK;
+}
+
+bool is_p2_supported(size_t n, int precision)
+{
+    if (n < 16 || (n & (n - 1))) return false;
+    return precision == 64 ? n <= 4096 : n <= 8192;
+}
+
+template <typename T, int N, bool LF, bool INV>
+static int launch_p2_inst(const void *in, void *out, size_t S, size_t nlines, const void *tw,
+                          double scale, hipStream_t s)
+{
+    using LP = P2Launch<T, N, LF>;
+    auto fn = k_fft_p2<T, N, LF, INV>;
+    static bool attr_done = false;  // one-time, benign race (idempotent)
+    if (!attr_done && LP::LDS_BYTES > 65536) {
+        QSIM_HIP(hipFuncSetAttribute((const void *)fn,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)LP::LDS_BYTES));
+    }
+    attr_done = true;
+    size_t blocks = (nlines + LP::L - 1) / LP::L;
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(LP::THREADS), LP::LDS_BYTES, s,
+                       (const cx<T> *)in, (cx<T> *)out, S, nlines, (const cx<T> *)tw, (T)scale);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+template <typename T, int N>
+static int launch_p2_n(bool lf, bool inv, const void *in, void *out, size_t S, size_t nlines,
+                       const void *tw, double scale, hipStream_t s)
+{
+    if (lf) {
+        return inv ? launch_p2_inst<T, N, true, true>(in, out, S, nlines, tw, scale, s)
+                   : launch_p2_inst<T, N, true, false>(in, out, S, nlines, tw, scale, s);
+    }
+    return inv ? launch_p2_inst<T, N, false, true>(in, out, S, nlines, tw, scale, s)
+               : launch_p2_inst<T, N, false, false>(in, out, S, nlines, tw, scale, s);
+}
+
+int launch_fft_p2(int precision, size_t n, bool inv, const void *in, void *out, size_t S,
+                  size_t nlines, const void *tw, double scale, hipStream_t s)
+{
+    const bool lf = S > 1;
+#define CASE_N(NN)                                                                     \
+    case NN:                                                                           \
+        return precision == 64                                                         \
+                   ? launch_p2_n<double, NN>(lf, inv, in, out, S, nlines, tw, scale, s) \
+                   : launch_p2_n<float, NN>(lf, inv, in, out, S, nlines, tw, scale, s);
+    switch (n) {
+        CASE_N(16)
+        CASE_N(32)
+        CASE_N(64)
+        CASE_N(128)
+        CASE_N(256)
+        CASE_N(512)
+        CASE_N(1024)
+        CASE_N(2048)
+        CASE_N(4096)
+        case 8192:
+            if (precision == 32)
+                return launch_p2_n<float, 8192>(lf, inv, in, out, S, nlines, tw, scale, s);
+            break;
+        default:
+            break;
+    }
+#undef CASE_N
+    return set_error(QSIM_ERR_UNSUPPORTED, "power-of-two FFT size not instantiated");
+}
+
+// Factor order 4, 2, 3, 5, 7, ... (the reference KISS plan, src/kissfft.hh:26-43).
+static void factorize(size_t n, GenFactors *gf)
+{
+    gf->n = (int)n;
+    gf->nf = 0;
+    size_t m = n, p = 4;
+    if (n <= 1) return;
+    do {
+        while (m % p) {
+            if (p == 4) p = 2;
+            else if (p == 2) p = 3;
+            else p += 2;
+            if (p * p > m) p = m;
+        }
+        m /= p;
+        gf->f[gf->nf++] = (int)p;
+    } while (m > 1 && gf->nf < 28);
+}
+
+int prepare_axis(FftAxis *a, size_t n, size_t S, size_t nlines, int precision)
+{
+    a->n = n;
+    a->S = S;
+    a->nlines = nlines;
+    a->p2 = is_p2_supported(n, precision);
+    QSIM_TRY(twiddle_table(n ? n : 1, precision, &a->tw));
+    if (a->p2) return QSIM_OK;
+    size_t esz = precision == 64 ? sizeof(c128) : sizeof(c64);
+    size_t per_line = 2 * n * esz;
+    const size_t lds_max = 160 * 1024;
+    if (per_line > lds_max)
+        return set_error(QSIM_ERR_UNSUPPORTED,
+                         "FFT length " + std::to_string(n) +
+                             " exceeds the single-pass LDS limit of this build");
+    int L = (int)((64 * 1024) / per_line);
+    if (L < 1) L = 1;
+    if (L > 16) L = 16;
+    if (S > 1) {  // a tile of adjacent lines: keep L a divisor of S
+        while (L > 1 && (S % L)) --L;
+    }
+    a->L = L;
+    a->lds = (size_t)L * per_line;
+    factorize(n, &a->gf);
+    return QSIM_OK;
+}
+
+int run_axis(const FftAxis &a, int precision, bool inv, const void *in, void *out, double scale,
+             hipStream_t s)
+{
+    // a length-1 axis is a copy/scale: the generic kernel with no factors
+    if (a.n == 1 && in == out && scale == 1.0) return QSIM_OK;
+    if (a.p2) return launch_fft_p2(precision, a.n, inv, in, out, a.S, a.nlines, a.tw, scale, s);
+    size_t blocks = (a.nlines + a.L - 1) / a.L;
+    static bool attr64 = false, attr32 = false;
+    if (precision == 64) {
+        if (!attr64) {
+            QSIM_HIP(hipFuncSetAttribute((const void *)k_fft_generic<double>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024));
+            attr64 = true;
+        }
+        hipLaunchKernelGGL(k_fft_generic<double>, dim3((unsigned)blocks), dim3(256), a.lds, s,
+                           (const c128 *)in, (c128 *)out, a.S, a.nlines, a.L, a.gf,
+                           (const c128 *)a.tw, (int)inv, scale);
+    } else {
+        if (!attr32) {
+            QSIM_HIP(hipFuncSetAttribute((const void *)k_fft_generic<float>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024));
+            attr32 = true;
+        }
+        hipLaunchKernelGGL(k_fft_generic<float>, dim3((unsigned)blocks), dim3(256), a.lds, s,
+                           (const c64 *)in, (c64 *)out, a.S, a.nlines, a.L, a.gf,
+                           (const c64 *)a.tw, (int)inv, (float)scale);
+    }
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+}  // namespace qsim
+
+using namespace qsim;
+
+struct qsim_fft_plan {
+    int rank = 0;
+    int inverse = 0;
+    int precision = 64;
+    size_t dims[3] = {1, 1, 1};
+    FftAxis axis[3];
+};
+
+extern "C" {
+
+int qsim_hip_fft_plan_create(qsim_fft_plan **plan, int rank, const size_t *dims, int inverse,
+                             int precision)
+{
+    QSIM_CHECK(plan, "null plan out-pointer");
+    *plan = nullptr;
+    QSIM_CHECK(rank >= 1 && rank <= 3, "FFT rank must be 1, 2 or 3");
+    QSIM_CHECK(dims, "null dims");
+    QSIM_CHECK(precision == 64 || precision == 32, "precision must be 64 or 32");
+    size_t total = 1;
+    for (int a = 0; a < rank; ++a) {
+        QSIM_CHECK(dims[a] >= 1, "FFT dimension must be >= 1");
+        total *= dims[a];
+    }
+    qsim_fft_plan *p = new qsim_fft_plan();
+    p->rank = rank;
+    p->inverse = inverse ? 1 : 0;
+    p->precision = precision;
+    for (int a = 0; a < rank; ++a) p->dims[a] = dims[a];
+    for (int a = 0; a < rank; ++a) {
+        size_t S = 1;
+        for (int b = a + 1; b < rank; ++b) S *= dims[b];
+        int rc = prepare_axis(&p->axis[a], dims[a], S, total / dims[a], precision);
+        if (rc != QSIM_OK) {
+            delete p;
+            return rc;
+        }
+    }
+    *plan = p;
+    return QSIM_OK;
+}
+
+int qsim_hip_fft_exec(qsim_fft_plan *p, const void *in, void *out, qsim_stream_t s)
+{
+    QSIM_CHECK(p && in && out, "null argument to qsim_hip_fft_exec");
+    hipStream_t st = as_stream(s);
+    // inner axis first, as the reference transforms (src/FourierTransform.cpp:248-284)
+    const void *src = in;
+    for (int a = p->rank - 1; a >= 0; --a) {
+        if (p->axis[a].n == 1 && src == out) continue;
+        QSIM_TRY(run_axis(p->axis[a], p->precision, p->inverse != 0, src, out, 1.0, st));
+        src = out;
+    }
+    if (src != out) {  // every axis had length 1
+        size_t bytes = (p->precision == 64 ? sizeof(c128) : sizeof(c64)) * p->dims[0] *
+                       p->dims[1] * p->dims[2];
+        QSIM_HIP(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    return QSIM_OK;
+}
+
+int qsim_hip_fft_plan_destroy(qsim_fft_plan *p)
+{
+    delete p;
+    return QSIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,223 @@
+// qsim_fft_core.h -- FFT building blocks for gfx950.
+//
+// * dft_reg<T, R, INV>: an R-point DFT (R = 2..64, power of two) held in one
+//   thread's registers: radix-2 decimation in frequency with compile-time
+//   twiddles, output permuted back to natural order by register renaming.
+// * Pow2Line<T, N, RA, RB, RC>: one length-N line transformed by N/RMAX
+//   threads in up to three Stockham passes (radices RA, RB, RC), data in
+//   registers, one LDS exchange between passes.  Forward runs RA, RB, RC;
+//   inverse runs RC, RB, RA, so the forward's outputs are exactly the
+//   inverse's inputs in every thread's registers (and vice versa): an
+//   FFT -> pointwise -> IFFT sequence needs no exchange around the pointwise
+//   step.
+//
+// Sign convention: forward X_k = sum x_n e^{-2 pi i k n / N}, inverse +,
+// unnormalised (reference src/FourierTransform.h:16-19).
+#pragma once
+
+#include "qsim_common.h"
+#include "twiddle_consts.h"
+
+namespace qsim {
+
+template <typename T>
+struct TwC;
+template <>
+struct TwC<double> {
+    static __device__ __forceinline__ double c(int k) { return kCos64d[k & 63]; }
+    static __device__ __forceinline__ double s(int k) { return kSin64d[k & 63]; }
+};
+template <>
+struct TwC<float> {
+    static __device__ __forceinline__ float c(int k) { return kCos64f[k & 63]; }
+    static __device__ __forceinline__ float s(int k) { return kSin64f[k & 63]; }
+};
+
+// a * exp(sgn * 2 pi i * k / m), sgn = +1 for INV, -1 forward; k, m are
+// compile-time after unrolling, so every branch folds away.
+template <typename T, bool INV>
+__device__ __forceinline__ cx<T> twmul(cx<T> a, int k, int m)
+{
+    const int q = (k * (64 / m)) & 63;  // angle 2 pi q / 64
+    if (q == 0) return a;
+    if (q == 32) return mk(-a.x, -a.y);
+    if (q == 16) return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);   // * (+-i)
+    if (q == 48) return INV ? mk(a.y, -a.x) : mk(-a.y, a.x);
+    const T c = TwC<T>::c(q);
+    const T s = INV ? TwC<T>::s(q) : -TwC<T>::s(q);
+    if ((q & 15) == 8) {  // |c| = |s| = sqrt(1/2)
+        const T h = c < 0 ? -c : c;
+        const T pc = c < 0 ? -a.x : a.x, qc = c < 0 ? -a.y : a.y;   // c*a / h
+        const T ps = s < 0 ? -a.y : a.y, qs = s < 0 ? -a.x : a.x;   // s*a / h
+        return mk(h * (pc - ps), h * (qc + qs));
+    }
+    return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+__host__ __device__ constexpr int brev(int x, int bits)
+{
+    int r = 0;
+    for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+// In-register R-point DFT, natural order in and out.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_reg(cx<T> *v)
+{
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2) {
+        cx<T> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if constexpr (R == 4) {
+        cx<T> a = v[0] + v[2], b = v[0] - v[2], c = v[1] + v[3], d = v[1] - v[3];
+        // forward: X1 = b - i d, X3 = b + i d
+        cx<T> id = INV ? mk(-d.y, d.x) : mk(d.y, -d.x);
+        v[0] = a + c;
+        v[2] = a - c;
+        v[1] = b + id;
+        v[3] = b - id;
+    } else {
+#pragma unroll
+        for (int m = R; m >= 2; m >>= 1) {
+            const int h = m >> 1;
+#pragma unroll
+            for (int b = 0; b < R; b += m) {
+#pragma unroll
+                for (int k = 0; k < h; ++k) {
+                    cx<T> a = v[b + k], c = v[b + k + h];
+                    v[b + k] = a + c;
+                    v[b + k + h] = twmul<T, INV>(a - c, k, m);
+                }
+            }
+        }
+        cx<T> t[R];
+        constexpr int bits = ilog2(R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) t[r] = v[brev(r, bits)];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = t[r];
+    }
+}
+
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+// One length-N line, N = RA*RB*RC (RB, RC may be 1).  Thread t < TPL holds
+// E = RMAX elements; for a pass of radix R it owns butterflies
+// j = t + u*TPL, u < E/R, element r of butterfly u in v[u*R + r].
+template <typename T, int N, int RA, int RB, int RC>
+struct Pow2Line {
+    static_assert(RA * RB * RC == N, "radices must multiply to N");
+    static constexpr int NP = RC > 1 ? 3 : (RB > 1 ? 2 : 1);
+    static constexpr int RA_ = RA;
+    static constexpr int E = cmax(RA, cmax(RB, RC));
+    static constexpr int TPL = N / E;
+    static constexpr int RMIN = cmin(RA, cmin(RB > 1 ? RB : RA, RC > 1 ? RC : RA));
+    // LDS elements per line: N plus one pad per RMIN, rounded to 1 mod 16 so
+    // lines placed side by side start on different banks.
+    static constexpr int LS_RAW = N + N / RMIN + 1;
+    static constexpr int LS = LS_RAW + ((1 - LS_RAW % 16) + 16) % 16;
+
+    template <int R>
+    static __device__ __forceinline__ int pad(int m) { return m + m / R; }
+
+    // Positions of the elements thread t holds as inputs of a first pass of
+    // radix R: element (u, r) is at j + r*(N/R).
+    template <int R>
+    static __device__ __forceinline__ int in_pos(int t, int u, int r)
+    {
+        return t + u * TPL + r * (N / R);
+    }
+
+    // twiddle + butterfly for a pass of radix R after NS points combined
+    template <int R, int NS, bool INV>
+    static __device__ __forceinline__ void pass(cx<T> *v, int t, const cx<T> *__restrict__ tw)
+    {
+#pragma unroll
+        for (int u = 0; u < E / R; ++u) {
+            if constexpr (NS > 1) {
+                const int j = t + u * TPL;
+                const int k = j & (NS - 1);
+                const int step = k * (N / (NS * R));
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cx<T> w = tw[step * r];
+                    v[u * R + r] = INV ? mulconj(v[u * R + r], w) : v[u * R + r] * w;
+                }
+            }
+            dft_reg<T, R, INV>(v + u * R);
+        }
+    }
+
+    // LDS exchange from the outputs of a pass (radix R, after NS) to the
+    // inputs of a pass of radix R2.
+    template <int R, int NS, int R2>
+    static __device__ __forceinline__ void exchange(cx<T> *v, int t, cx<T> *lds)
+    {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < E / R; ++u) {
+            const int j = t + u * TPL;
+            const int base = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[pad<R>(base + r * NS)] = v[u * R + r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < E / R2; ++u)
+#pragma unroll
+            for (int r = 0; r < R2; ++r) v[u * R2 + r] = lds[pad<R>(in_pos<R2>(t, u, r))];
+    }
+
+    // Forward transform.  In: inputs of radix-RA pass (in_pos<RA>).  Out: X at
+    // in_pos<R_last>(t, u, r) with R_last = RC, RB or RA (NP = 3, 2, 1).
+    static __device__ __forceinline__ void forward(cx<T> *v, int t, cx<T> *lds,
+                                                   const cx<T> *__restrict__ tw)
+    {
+        pass<RA, 1, false>(v, t, tw);
+        if constexpr (NP >= 2) {
+            exchange<RA, 1, RB>(v, t, lds);
+            pass<RB, RA, false>(v, t, tw);
+        }
+        if constexpr (NP == 3) {
+            exchange<RB, RA, RC>(v, t, lds);
+            pass<RC, RA * RB, false>(v, t, tw);
+        }
+    }
+
+    static constexpr int RLAST = NP == 3 ? RC : (NP == 2 ? RB : RA);
+
+    // Inverse transform, radices RLAST, ..., RA.  In: in_pos<RLAST>.  Out: x at
+    // in_pos<RA>(t, u, r).
+    static __device__ __forceinline__ void inverse(cx<T> *v, int t, cx<T> *lds,
+                                                   const cx<T> *__restrict__ tw)
+    {
+        if constexpr (NP == 1) {
+            pass<RA, 1, true>(v, t, tw);
+        } else if constexpr (NP == 2) {
+            pass<RB, 1, true>(v, t, tw);
+            exchange<RB, 1, RA>(v, t, lds);
+            pass<RA, RB, true>(v, t, tw);
+        } else {
+            pass<RC, 1, true>(v, t, tw);
+            exchange<RC, 1, RB>(v, t, lds);
+            pass<RB, RC, true>(v, t, tw);
+            exchange<RB, RC, RA>(v, t, lds);
+            pass<RA, RC * RB, true>(v, t, tw);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Line addressing: a tensor viewed as [outer][N][S]; line g (g < outer*S) is
+// (o = g / S, c = g % S), element m at o*N*S + c + m*S.
+__device__ __forceinline__ size_t line_base(size_t g, size_t N, size_t S)
+{
+    return (g / S) * N * S + (g % S);
+}
+
+}  // namespace qsim

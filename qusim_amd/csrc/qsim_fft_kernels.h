@@ -1,0 +1,169 @@
+// qsim_fft_kernels.h -- FFT kernels over the lines of one tensor axis.
+#pragma once
+
+#include "qsim_fft_core.h"
+
+namespace qsim {
+
+// Radix plan of the register-resident power-of-two line FFT, per size and
+// element type.  N/E threads per line, E = largest radix.
+template <typename T, int N>
+struct P2Cfg;
+#define QSIM_P2CFG(TYPE, N_, A, B, C_)                 \
+    template <>                                        \
+    struct P2Cfg<TYPE, N_> {                           \
+        using Line = Pow2Line<TYPE, N_, A, B, C_>;     \
+    };
+QSIM_P2CFG(double, 16, 4, 4, 1)
+QSIM_P2CFG(double, 32, 8, 4, 1)
+QSIM_P2CFG(double, 64, 8, 8, 1)
+QSIM_P2CFG(double, 128, 16, 8, 1)
+QSIM_P2CFG(double, 256, 16, 16, 1)
+QSIM_P2CFG(double, 512, 8, 8, 8)
+QSIM_P2CFG(double, 1024, 16, 16, 4)
+QSIM_P2CFG(double, 2048, 16, 16, 8)
+QSIM_P2CFG(double, 4096, 16, 16, 16)
+QSIM_P2CFG(float, 16, 4, 4, 1)
+QSIM_P2CFG(float, 32, 8, 4, 1)
+QSIM_P2CFG(float, 64, 8, 8, 1)
+QSIM_P2CFG(float, 128, 16, 8, 1)
+QSIM_P2CFG(float, 256, 16, 16, 1)
+QSIM_P2CFG(float, 512, 16, 32, 1)
+QSIM_P2CFG(float, 1024, 32, 32, 1)
+QSIM_P2CFG(float, 2048, 16, 16, 8)
+QSIM_P2CFG(float, 4096, 16, 16, 16)
+QSIM_P2CFG(float, 8192, 16, 16, 32)
+#undef QSIM_P2CFG
+
+// Lines per workgroup: rows (position-fast) fill ~256 threads; strided axes
+// (line-fast) take 128 B of adjacent lines per row of the tile.
+template <typename T, int N, bool LF>
+struct P2Launch {
+    using Line = typename P2Cfg<T, N>::Line;
+    static constexpr int LF_LINES = 128 / (int)sizeof(cx<T>);
+    static constexpr int ROWS = cmax(1, 256 / Line::TPL);
+    static constexpr int L_WANT = LF ? LF_LINES : ROWS;
+    // keep the LDS tile within 96 KiB
+    static constexpr int L_LDS = cmax(1, (96 * 1024) / (Line::LS * (int)sizeof(cx<T>)));
+    static constexpr int L = cmin(L_WANT, L_LDS);
+    static constexpr int THREADS = L * Line::TPL;
+    static constexpr size_t LDS_BYTES = (size_t)L * Line::LS * sizeof(cx<T>);
+};
+
+// Plain transform of `nlines` lines of length N with stride S (in may be out).
+template <typename T, int N, bool LF, bool INV>
+__global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
+    k_fft_p2(const cx<T> *in, cx<T> *out, size_t S, size_t nlines,
+             const cx<T> *__restrict__ tw, T scale)
+{
+    using LP = P2Launch<T, N, LF>;
+    using P = typename LP::Line;
+    constexpr int L = LP::L;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int l = LF ? tid % L : tid / P::TPL;
+    const int t = LF ? tid / L : tid % P::TPL;
+    const size_t g = (size_t)blockIdx.x * L + l;
+    const bool active = g < nlines;
+    const size_t base = active ? line_base(g, N, S) : 0;
+    constexpr int R0 = INV ? P::RLAST : P::RA_;
+    constexpr int R1 = INV ? P::RA_ : P::RLAST;
+    cx<T> v[P::E];
+#pragma unroll
+    for (int u = 0; u < P::E / R0; ++u)
+#pragma unroll
+        for (int r = 0; r < R0; ++r)
+            v[u * R0 + r] = active ? in[base + (size_t)P::template in_pos<R0>(t, u, r) * S]
+                                   : mk<T>(0, 0);
+    if constexpr (INV)
+        P::inverse(v, t, lds + l * P::LS, tw);
+    else
+        P::forward(v, t, lds + l * P::LS, tw);
+    if (active) {
+#pragma unroll
+        for (int u = 0; u < P::E / R1; ++u)
+#pragma unroll
+            for (int r = 0; r < R1; ++r)
+                out[base + (size_t)P::template in_pos<R1>(t, u, r) * S] = v[u * R1 + r] * scale;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic mixed-radix Stockham for any N whose line fits in LDS twice.  Each
+// output of a radix-R pass is its own R-term DFT sum with table twiddles, so
+// every factor (2, 3, 5, 7, ..., large primes) takes the same code path.
+struct GenFactors {
+    int n;
+    int nf;
+    int f[28];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_fft_generic(const cx<T> *in, cx<T> *out, size_t S, size_t nlines, int L, GenFactors gf,
+                  const cx<T> *__restrict__ tw, int inverse, T scale)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int N = gf.n;
+    cx<T> *src = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *dst = src + (size_t)L * N;
+    const size_t g0 = (size_t)blockIdx.x * L;
+    const int total = L * N;
+    for (int o = threadIdx.x; o < total; o += blockDim.x) {
+        int l, m;
+        if (S == 1) {
+            l = o / N;
+            m = o - l * N;
+        } else {
+            l = o % L;
+            m = o / L;
+        }
+        size_t g = g0 + l;
+        if (g < nlines) src[l * N + m] = in[line_base(g, N, S) + (size_t)m * S];
+    }
+    __syncthreads();
+    int ns = 1;
+    for (int p = 0; p < gf.nf; ++p) {
+        const int R = gf.f[p];
+        const int NR = N / R;
+        for (int o = threadIdx.x; o < total; o += blockDim.x) {
+            const int l = o / N, pos = o - l * N;
+            const int k = pos % ns;
+            const int r = (pos / ns) % R;
+            const int jb = pos / (ns * R);
+            const int j = jb * ns + k;
+            const int mstep = k * (N / (ns * R)) + r * NR;
+            const cx<T> *s = src + l * N + j;
+            cx<T> acc = s[0];
+            int e = 0;
+            for (int q = 1; q < R; ++q) {
+                e += mstep;
+                if (e >= N) e -= N;
+                cx<T> w = tw[e];
+                cx<T> x = s[q * NR];
+                acc = acc + (inverse ? mulconj(x, w) : x * w);
+            }
+            dst[l * N + pos] = acc;
+        }
+        __syncthreads();
+        cx<T> *tmp = src;
+        src = dst;
+        dst = tmp;
+        ns *= R;
+    }
+    for (int o = threadIdx.x; o < total; o += blockDim.x) {
+        int l, m;
+        if (S == 1) {
+            l = o / N;
+            m = o - l * N;
+        } else {
+            l = o % L;
+            m = o / L;
+        }
+        size_t g = g0 + l;
+        if (g < nlines) out[line_base(g, N, S) + (size_t)m * S] = src[l * N + m] * scale;
+    }
+}
+
+}  // namespace qsim

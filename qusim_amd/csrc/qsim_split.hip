@@ -1,0 +1,833 @@
+// qsim_split.hip -- split-operator evolution job (SplittingMethod1D/2D on
+// gfx950).
+//
+// Reference step (src/SplittingUtils.h:14-64, src/SplittingMethod2D.cpp:81-121):
+//   psi <- E_V psi ; psi <- IFFT(E_T FFT(psi)) / N ; psi <- E_V psi
+// Fused schedule (2D, power-of-two axes): the potential half steps of
+// consecutive steps meet in real space, so with FFT2 = FFT_x FFT_y a run of
+// steps becomes alternating sweeps, each one read + one write of psi:
+//   row sweep  (y lines, contiguous): IFFT_y -> x E_V(x,y) -> FFT_y
+//   col sweep  (x lines, stride ny):  FFT_x  -> x Tx(kx) Ty(ky) -> IFFT_x
+// with E_T = Tx Ty separable and 1/(nx ny) folded into Tx.  The first sweep
+// only applies E_V then FFT_y; the last only IFFT_y then E_V.
+// 1D (power-of-two): one workgroup per job keeps psi in registers for all
+// steps.  Other sizes and the infinite-wall boundary run the reference's op
+// sequence unfused (FFT plans + pointwise kernels).
+//
+// Precision 64: complex128 everywhere.  Precision 32 (FP32-mixed): psi is
+// stored and transformed in complex64; every phase factor is built in FP64
+// and applied as hi + lo FP32 parts, so the rounding of a factor is never
+// repeated step after step (only the product is rounded).
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "qsim_fft_kernels.h"
+#include "qsim_fft_plan.h"
+
+namespace qsim {
+
+enum { SW_FIRST = 0, SW_MID = 1, SW_LAST = 2 };
+
+// Device-side description of one job (the kernels index a table of these by
+// blockIdx.y, so many same-shape jobs share one launch).
+struct SplitDev {
+    c128 *psi;             // caller's complex128 state
+    void *work;            // working state (== psi for precision 64)
+    const c128 *ev[4];     // potential factors over the grid
+    const c128 *kx[2];     // kinetic factor along x (2D) or the whole line (1D)
+    const c128 *ky[2];     // kinetic factor along y (2D)
+};
+
+// v * f with f an FP64 factor: FP64 multiply, or for FP32 data the factor
+// split into hi + lo FP32 parts.
+__device__ __forceinline__ c128 apply(c128 v, c128 f) { return v * f; }
+__device__ __forceinline__ c64 apply(c64 v, c128 f)
+{
+    c64 hi = cvt<float>(f);
+    c64 lo = cvt<float>(mk(f.x - (double)hi.x, f.y - (double)hi.y));
+    return v * hi + v * lo;
+}
+
+template <typename TO, typename TI>
+__device__ __forceinline__ cx<TO> conv(cx<TI> a)
+{
+    return cvt<TO>(a);
+}
+
+// ---------------------------------------------------------------- 2D sweeps
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((P2Launch<T, N, false>::THREADS))
+    k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
+                const cx<T> *__restrict__ tw)
+{
+    using LP = P2Launch<T, N, false>;
+    using P = typename LP::Line;
+    constexpr int L = LP::L;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const SplitDev &J = jobs[blockIdx.y];
+    const int tid = threadIdx.x;
+    const int l = tid / P::TPL, t = tid % P::TPL;
+    const size_t g = (size_t)blockIdx.x * L + l;
+    const bool active = g < nrows;
+    const size_t base = (active ? g : 0) * N;
+    const c128 *__restrict__ ev = J.ev[iev];
+    cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
+    constexpr int RA = P::RA_, RZ = P::RLAST;
+    cx<T> v[P::E];
+    if constexpr (MODE == SW_FIRST) {
+#pragma unroll
+        for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+            for (int r = 0; r < RA; ++r) {
+                const int pos = P::template in_pos<RA>(t, u, r);
+                v[u * RA + r] =
+                    active ? apply(conv<T>(J.psi[base + pos]), ev[base + pos]) : mk<T>(0, 0);
+            }
+    } else {
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                v[u * RZ + r] =
+                    active ? work[base + P::template in_pos<RZ>(t, u, r)] : mk<T>(0, 0);
+        P::inverse(v, t, lds + l * P::LS, tw);
+#pragma unroll
+        for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+            for (int r = 0; r < RA; ++r) {
+                const int pos = P::template in_pos<RA>(t, u, r);
+                if (active) v[u * RA + r] = apply(v[u * RA + r], ev[base + pos]);
+            }
+    }
+    if constexpr (MODE == SW_LAST) {
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                for (int r = 0; r < RA; ++r)
+                    J.psi[base + P::template in_pos<RA>(t, u, r)] = conv<double>(v[u * RA + r]);
+        }
+    } else {
+        P::forward(v, t, lds + l * P::LS, tw);
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+                for (int r = 0; r < RZ; ++r)
+                    work[base + P::template in_pos<RZ>(t, u, r)] = v[u * RZ + r];
+        }
+    }
+}
+
+// Column sweep over ncols columns of an nx(=N) x ncols array.
+template <typename T, int N>
+__global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
+    k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
+                const cx<T> *__restrict__ tw)
+{
+    using LP = P2Launch<T, N, true>;
+    using P = typename LP::Line;
+    constexpr int L = LP::L;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const SplitDev &J = jobs[blockIdx.y];
+    const int tid = threadIdx.x;
+    const int l = tid % L, t = tid / L;
+    const size_t j = (size_t)blockIdx.x * L + l;
+    const bool active = j < ncols;
+    const size_t c = active ? j : 0;
+    cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
+    const c128 *__restrict__ kx = J.kx[ik];
+    const c128 kyj = J.ky[ik][c];
+    constexpr int RA = P::RA_, RZ = P::RLAST;
+    cx<T> v[P::E];
+#pragma unroll
+    for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+        for (int r = 0; r < RA; ++r)
+            v[u * RA + r] =
+                active ? work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols] : mk<T>(0, 0);
+    P::forward(v, t, lds + l * P::LS, tw);
+#pragma unroll
+    for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) {
+            const int k = P::template in_pos<RZ>(t, u, r);
+            v[u * RZ + r] = apply(v[u * RZ + r], kx[k] * kyj);
+        }
+    P::inverse(v, t, lds + l * P::LS, tw);
+    if (active) {
+#pragma unroll
+        for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+            for (int r = 0; r < RA; ++r)
+                work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols] = v[u * RA + r];
+    }
+}
+
+// ---------------------------------------------------------------- 1D job
+// Phase program: start with ev[ev0]; each step runs (kinetic k[i], potential
+// ev[e[i]]) for i < nph; the last potential factor of the last step is
+// ev[ev_end] instead.
+struct Prog1D {
+    int ev0, ev_end, nph;
+    int k[3], e[3];
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__((P2Cfg<T, N>::Line::TPL))
+    k_split1d(const SplitDev *__restrict__ jobs, Prog1D prog, size_t nsteps,
+              const cx<T> *__restrict__ tw)
+{
+    using P = typename P2Cfg<T, N>::Line;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const SplitDev &J = jobs[blockIdx.x];
+    const int t = threadIdx.x;
+    constexpr int RA = P::RA_, RZ = P::RLAST;
+    cx<T> v[P::E];
+    const c128 *__restrict__ e0 = J.ev[prog.ev0];
+#pragma unroll
+    for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+        for (int r = 0; r < RA; ++r) {
+            const int pos = P::template in_pos<RA>(t, u, r);
+            v[u * RA + r] = apply(conv<T>(J.psi[pos]), e0[pos]);
+        }
+    for (size_t s = 0; s < nsteps; ++s) {
+        for (int ph = 0; ph < prog.nph; ++ph) {
+            P::forward(v, t, lds, tw);
+            const c128 *__restrict__ kt = J.kx[prog.k[ph]];
+#pragma unroll
+            for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) {
+                    const int k = P::template in_pos<RZ>(t, u, r);
+                    v[u * RZ + r] = apply(v[u * RZ + r], kt[k]);
+                }
+            P::inverse(v, t, lds, tw);
+            const bool last = (s + 1 == nsteps) && (ph + 1 == prog.nph);
+            const c128 *__restrict__ ev = J.ev[last ? prog.ev_end : prog.e[ph]];
+#pragma unroll
+            for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                for (int r = 0; r < RA; ++r) {
+                    const int pos = P::template in_pos<RA>(t, u, r);
+                    v[u * RA + r] = apply(v[u * RA + r], ev[pos]);
+                }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+        for (int r = 0; r < RA; ++r)
+            J.psi[P::template in_pos<RA>(t, u, r)] = conv<double>(v[u * RA + r]);
+}
+
+// ---------------------------------------------------------------- helpers
+template <typename F>
+__global__ void __launch_bounds__(256) k_each(size_t n, F f)
+{
+    size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) f(i);
+}
+
+template <typename F>
+int each(size_t n, hipStream_t s, F f)
+{
+    if (!n) return QSIM_OK;
+    size_t b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_each<F>, dim3((unsigned)b), dim3(256), 0, s, n, f);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+using cd = std::complex<double>;
+
+}  // namespace qsim
+
+using namespace qsim;
+
+struct qsim_split_job {
+    qsim_split_desc d{};
+    size_t n = 0;
+    bool fused = false;
+    c128 *psi = nullptr;
+    void *work = nullptr;   // owned when precision 32
+    std::vector<void *> owned;
+    SplitDev host{};
+    SplitDev *dev = nullptr;  // device copy of `host`
+    const void *tw_x = nullptr, *tw_y = nullptr;  // twiddles of the sweep kernels
+    // phase program
+    int ev_start = 0, ev_end = 0, nph = 0;
+    int ph_k[3] = {0, 0, 0}, ph_e[3] = {0, 0, 0};
+    // unfused path
+    qsim_fft_plan *fwd = nullptr, *inv = nullptr;
+    c128 *tmp = nullptr, *tmp2 = nullptr;
+    c128 *et_full[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+// O4 coefficients, src/SplittingUtils.cpp:3-13
+struct O4 {
+    double C1, C2, D1, D2;
+    O4()
+    {
+        double t = std::pow(2, 1 / 3.0);
+        C1 = 1 / (2 * (2 - t));
+        C2 = (1 - t) / (2 * (2 - t));
+        D1 = 1 / (2 - t);
+        D2 = -t / (2 - t);
+    }
+};
+
+int dalloc(qsim_split_job *j, void **p, size_t bytes)
+{
+    QSIM_HIP(hipMalloc(p, bytes));
+    j->owned.push_back(*p);
+    return QSIM_OK;
+}
+
+// ev = exp(f V), f = -I/hbar * dt * t (src/SplittingMethod2D.cpp:43-52)
+int build_ev(qsim_split_job *j, c128 **out, double t, const double *Vre, const double *Vim,
+             hipStream_t s)
+{
+    void *p;
+    QSIM_TRY(dalloc(j, &p, j->n * sizeof(c128)));
+    c128 *ev = (c128 *)p;
+    cd f = cd(-0.0, -1.0) / j->d.hbar * cd(j->d.dt_re, j->d.dt_im) * t;
+    c128 fc = mk(f.real(), f.imag());
+    QSIM_TRY(each(j->n, s, [=] __device__(size_t i) {
+        c128 z = Vim ? fc * mk(Vre[i], Vim[i]) : fc * Vre[i];
+        ev[i] = cexp(z);
+    }));
+    *out = ev;
+    return QSIM_OK;
+}
+
+int square_ev(qsim_split_job *j, c128 **out, const c128 *a, hipStream_t s)
+{
+    void *p;
+    QSIM_TRY(dalloc(j, &p, j->n * sizeof(c128)));
+    c128 *ev = (c128 *)p;
+    QSIM_TRY(each(j->n, s, [=] __device__(size_t i) { ev[i] = a[i] * a[i]; }));
+    *out = ev;
+    return QSIM_OK;
+}
+
+// exp(alpha * a * fold(i)^2) * scale over one axis of length n (host, FP64)
+int build_axis(qsim_split_job *j, c128 **out, size_t n, cd alpha, double a, double scale)
+{
+    std::vector<c128> h(n);
+    for (size_t i = 0; i < n; ++i) {
+        double k = (double)fold_half(i, n);
+        cd e = std::exp(alpha * (k * k * a)) * scale;
+        h[i] = mk(e.real(), e.imag());
+    }
+    void *p;
+    QSIM_TRY(dalloc(j, &p, n * sizeof(c128)));
+    QSIM_HIP(hipMemcpy(p, h.data(), n * sizeof(c128), hipMemcpyHostToDevice));
+    *out = (c128 *)p;
+    return QSIM_OK;
+}
+
+template <typename T, int N, int MODE>
+int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const void *tw,
+               hipStream_t s)
+{
+    using LP = P2Launch<T, N, false>;
+    auto fn = k_row_sweep<T, N, MODE>;
+    if (LP::LDS_BYTES > 65536)
+        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)LP::LDS_BYTES));
+    dim3 grid((unsigned)((nrows + LP::L - 1) / LP::L), (unsigned)njobs);
+    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, nrows, iev,
+                       (const cx<T> *)tw);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+template <typename T, int N>
+int launch_col(const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
+               hipStream_t s)
+{
+    using LP = P2Launch<T, N, true>;
+    auto fn = k_col_sweep<T, N>;
+    if (LP::LDS_BYTES > 65536)
+        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)LP::LDS_BYTES));
+    dim3 grid((unsigned)((ncols + LP::L - 1) / LP::L), (unsigned)njobs);
+    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, ncols, ik,
+                       (const cx<T> *)tw);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+template <typename T, int N>
+int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps, const void *tw,
+              hipStream_t s)
+{
+    using P = typename P2Cfg<T, N>::Line;
+    auto fn = k_split1d<T, N>;
+    const size_t lds = (size_t)P::LS * sizeof(cx<T>);
+    if (lds > 65536)
+        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+    hipLaunchKernelGGL(fn, dim3((unsigned)njobs), dim3(P::TPL), lds, s, jobs, prog, nsteps,
+                       (const cx<T> *)tw);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+// dispatch on (precision, length); returns UNSUPPORTED for lengths without a
+// fused instance
+#define QSIM_SWITCH_N(PREC, NVAL, CALL)                                               \
+    switch (NVAL) {                                                                   \
+        case 64: return PREC == 64 ? CALL(double, 64) : CALL(float, 64);              \
+        case 128: return PREC == 64 ? CALL(double, 128) : CALL(float, 128);           \
+        case 256: return PREC == 64 ? CALL(double, 256) : CALL(float, 256);           \
+        case 512: return PREC == 64 ? CALL(double, 512) : CALL(float, 512);           \
+        case 1024: return PREC == 64 ? CALL(double, 1024) : CALL(float, 1024);        \
+        case 2048: return PREC == 64 ? CALL(double, 2048) : CALL(float, 2048);        \
+        case 4096: return PREC == 64 ? CALL(double, 4096) : CALL(float, 4096);        \
+        case 8192:                                                                    \
+            if (PREC == 32) return CALL(float, 8192);                                 \
+            break;                                                                    \
+        default: break;                                                               \
+    }                                                                                 \
+    return set_error(QSIM_ERR_UNSUPPORTED, "no fused kernel for this length");
+
+int row_sweep(int prec, size_t ny, int mode, const SplitDev *jobs, int njobs, size_t nrows,
+              int iev, const void *tw, hipStream_t s)
+{
+#define ROWCALL(T, NN)                                                                     \
+    (mode == SW_FIRST ? launch_row<T, NN, SW_FIRST>(jobs, njobs, nrows, iev, tw, s)         \
+                      : mode == SW_MID ? launch_row<T, NN, SW_MID>(jobs, njobs, nrows, iev, tw, s) \
+                                       : launch_row<T, NN, SW_LAST>(jobs, njobs, nrows, iev, tw, s))
+    QSIM_SWITCH_N(prec, ny, ROWCALL)
+#undef ROWCALL
+}
+
+int col_sweep(int prec, size_t nx, const SplitDev *jobs, int njobs, size_t ncols, int ik,
+              const void *tw, hipStream_t s)
+{
+#define COLCALL(T, NN) launch_col<T, NN>(jobs, njobs, ncols, ik, tw, s)
+    QSIM_SWITCH_N(prec, nx, COLCALL)
+#undef COLCALL
+}
+
+int run_1d(int prec, size_t n, const SplitDev *jobs, int njobs, const Prog1D &prog,
+           size_t nsteps, const void *tw, hipStream_t s)
+{
+#define ONECALL(T, NN) launch_1d<T, NN>(jobs, njobs, prog, nsteps, tw, s)
+    QSIM_SWITCH_N(prec, n, ONECALL)
+#undef ONECALL
+}
+
+bool fused_len_ok(size_t n, int prec)
+{
+    if (n < 64 || (n & (n - 1))) return false;
+    return prec == 64 ? n <= 4096 : n <= 8192;
+}
+
+// Optional per-kernel timing: HIP events around every sweep launch.
+struct SweepTimer {
+    std::vector<hipEvent_t> ev;  // pairs (start, stop)
+    std::vector<int> kind;       // 0 row (incl. first/last), 1 column, 2 1D
+    int mark(int k, hipStream_t s)
+    {
+        hipEvent_t e;
+        QSIM_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+        if (k >= 0) kind.push_back(k);
+        QSIM_HIP(hipEventRecord(e, s));
+        return QSIM_OK;
+    }
+    ~SweepTimer()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+#define TIMED(T, K, CALL)                          \
+    do {                                           \
+        if (T) QSIM_TRY((T)->mark((K), s));        \
+        QSIM_TRY(CALL);                            \
+        if (T) QSIM_TRY((T)->mark(-1, s));         \
+    } while (0)
+
+// ------------------------------------------------- fused schedule (2D / 1D)
+int fused_steps(qsim_split_job *j, const SplitDev *jobs, int njobs, size_t nsteps,
+                hipStream_t s, SweepTimer *tm = nullptr)
+{
+    const int prec = j->d.precision;
+    if (j->d.dim == 1) {
+        Prog1D p;
+        p.ev0 = j->ev_start;
+        p.ev_end = j->ev_end;
+        p.nph = j->nph;
+        for (int i = 0; i < 3; ++i) {
+            p.k[i] = j->ph_k[i];
+            p.e[i] = j->ph_e[i];
+        }
+        TIMED(tm, 2, run_1d(prec, j->d.nx, jobs, njobs, p, nsteps, j->tw_x, s));
+        return QSIM_OK;
+    }
+    const size_t nx = j->d.nx, ny = j->d.ny;
+    TIMED(tm, 0, row_sweep(prec, ny, SW_FIRST, jobs, njobs, nx, j->ev_start, j->tw_y, s));
+    for (size_t st = 0; st < nsteps; ++st) {
+        for (int ph = 0; ph < j->nph; ++ph) {
+            TIMED(tm, 1, col_sweep(prec, nx, jobs, njobs, ny, j->ph_k[ph], j->tw_x, s));
+            const bool last = (st + 1 == nsteps) && (ph + 1 == j->nph);
+            if (last)
+                TIMED(tm, 0, row_sweep(prec, ny, SW_LAST, jobs, njobs, nx, j->ev_end, j->tw_y, s));
+            else
+                TIMED(tm, 0, row_sweep(prec, ny, SW_MID, jobs, njobs, nx, j->ph_e[ph], j->tw_y, s));
+        }
+    }
+    return QSIM_OK;
+}
+
+// ------------------------------------------------- unfused reference order
+// ExpV: psi *= ev (src/SplittingMethod2D.cpp:81-94); ExpT: FFT, *= E_T, IFFT,
+// *1/N (:96-121).  1D InfiniteWall: src/SplittingMethod1D.cpp:62-81.
+int unfused_expv(qsim_split_job *j, const c128 *ev, hipStream_t s)
+{
+    c128 *psi = j->psi;
+    return each(j->n, s, [=] __device__(size_t i) { psi[i] = ev[i] * psi[i]; });
+}
+
+int unfused_expt(qsim_split_job *j, int ik, hipStream_t s)
+{
+    const size_t n = j->n;
+    c128 *psi = j->psi, *ft = j->tmp;
+    if (j->d.boundary == 0) {
+        QSIM_TRY(qsim_hip_fft_exec(j->fwd, psi, ft, s));
+        const c128 *et = j->et_full[ik];
+        QSIM_TRY(each(n, s, [=] __device__(size_t i) { ft[i] = et[i] * ft[i]; }));
+        QSIM_TRY(qsim_hip_fft_exec(j->inv, ft, psi, s));
+        const double sc = 1. / (double)n;
+        return each(n, s, [=] __device__(size_t i) { psi[i] = sc * psi[i]; });
+    }
+    // infinite wall: odd extension to 2N, sine transform by the 2N inverse DFT
+    const size_t N = n;
+    c128 *a = j->tmp, *b = j->tmp2;
+    QSIM_TRY(each(2 * N, s, [=] __device__(size_t i) {
+        c128 z;
+        if (i == 0 || i == N) z = mk(0.0, 0.0);
+        else if (i < N) z = psi[i];
+        else z = mk(-psi[2 * N - i].x, -psi[2 * N - i].y);
+        a[i] = z;
+    }));
+    QSIM_TRY(qsim_hip_fft_exec(j->inv, a, b, s));
+    const c128 *et = j->et_full[ik];  // exp(f i^2), i < N
+    QSIM_TRY(each(N, s, [=] __device__(size_t i) { b[i] = b[i] * et[i]; }));
+    QSIM_TRY(each(2 * N, s, [=] __device__(size_t i) {
+        if (i == 0 || i == N) b[i] = mk(0.0, 0.0);
+        else if (i > N) b[i] = mk(-b[2 * N - i].x, -b[2 * N - i].y);
+    }));
+    QSIM_TRY(qsim_hip_fft_exec(j->inv, b, a, s));
+    const double sc = -1. / (2. * N);
+    return each(N, s, [=] __device__(size_t i) { psi[i] = sc * a[i]; });
+}
+
+int unfused_steps(qsim_split_job *j, size_t nsteps, hipStream_t s)
+{
+    const c128 *const *ev = j->host.ev;
+    const bool o4 = j->d.method == 4;
+    for (size_t st = 0; st < nsteps; ++st) {
+        if (!o4) {
+            QSIM_TRY(unfused_expv(j, ev[0], s));
+            QSIM_TRY(unfused_expt(j, 0, s));
+            QSIM_TRY(unfused_expv(j, ev[0], s));
+        } else {
+            QSIM_TRY(unfused_expv(j, ev[0], s));
+            QSIM_TRY(unfused_expt(j, 0, s));
+            QSIM_TRY(unfused_expv(j, ev[1], s));
+            QSIM_TRY(unfused_expt(j, 1, s));
+            QSIM_TRY(unfused_expv(j, ev[1], s));
+            QSIM_TRY(unfused_expt(j, 0, s));
+            QSIM_TRY(unfused_expv(j, ev[0], s));
+        }
+    }
+    return QSIM_OK;
+}
+
+void destroy(qsim_split_job *j)
+{
+    if (!j) return;
+    for (void *p : j->owned) (void)hipFree(p);
+    if (j->fwd) qsim_hip_fft_plan_destroy(j->fwd);
+    if (j->inv) qsim_hip_fft_plan_destroy(j->inv);
+    delete j;
+}
+
+int create(qsim_split_job *j, const double *Vre, const double *Vim)
+{
+    const qsim_split_desc &d = j->d;
+    hipStream_t s = nullptr;
+    const O4 c;
+    const bool o4 = d.method == 4;
+    const double dx = (d.x1 - d.x0) / d.nx;
+    const double dy = d.dim == 2 ? (d.y1 - d.y0) / d.ny : 1.0;
+    const cd dt(d.dt_re, d.dt_im);
+    j->fused = d.boundary == 0 && fused_len_ok(d.nx, d.precision) &&
+               (d.dim == 1 || fused_len_ok(d.ny, d.precision));
+    if (!j->fused && d.precision != 64)
+        return set_error(QSIM_ERR_UNSUPPORTED,
+                         "FP32-mixed split job needs power-of-two axes (64..8192) and a "
+                         "periodic boundary");
+    // potential factors
+    SplitDev &h = j->host;
+    h.psi = j->psi;
+    c128 *ev0, *ev1, *ev2;
+    if (!o4) {
+        QSIM_TRY(build_ev(j, &ev0, 0.5, Vre, Vim, s));
+        h.ev[0] = ev0;
+        if (j->fused) {
+            QSIM_TRY(square_ev(j, &ev1, ev0, s));
+            h.ev[1] = ev1;
+        }
+        j->ev_start = 0;
+        j->ev_end = 0;
+        j->nph = 1;
+        j->ph_k[0] = 0;
+        j->ph_e[0] = 1;
+    } else {
+        QSIM_TRY(build_ev(j, &ev0, c.C1, Vre, Vim, s));
+        QSIM_TRY(build_ev(j, &ev1, c.C2, Vre, Vim, s));
+        h.ev[0] = ev0;
+        h.ev[1] = ev1;
+        if (j->fused) {
+            QSIM_TRY(square_ev(j, &ev2, ev0, s));
+            h.ev[2] = ev2;
+        }
+        j->ev_start = 0;
+        j->ev_end = 0;
+        j->nph = 3;
+        j->ph_k[0] = 0;  // D1
+        j->ph_e[0] = 1;  // C2
+        j->ph_k[1] = 1;  // D2
+        j->ph_e[1] = 1;  // C2
+        j->ph_k[2] = 0;  // D1
+        j->ph_e[2] = 2;  // 2 C1 (merged across steps)
+    }
+    // kinetic factors
+    const double Dkx = 2 * M_PI / (dx * d.nx);
+    const double DTx = (Dkx * d.hbar) * (Dkx * d.hbar) / (2 * d.mass);
+    const double coef[2] = {o4 ? c.D1 : 1.0, o4 ? c.D2 : 0.0};
+    const int nk = o4 ? 2 : 1;
+    if (d.dim == 2) {
+        const double Dky = 2 * M_PI / (dy * d.ny);
+        const double DTy = (Dky * d.hbar) * (Dky * d.hbar) / (2 * d.mass);
+        const cd alpha = cd(-0.0, -1.0) * dt / d.hbar;  // src/SplittingMethod2D.cpp:63
+        for (int k = 0; k < nk; ++k) {
+            c128 *tx, *ty;
+            QSIM_TRY(build_axis(j, &tx, d.nx, alpha * coef[k], DTx, 1.0 / (double)j->n));
+            QSIM_TRY(build_axis(j, &ty, d.ny, alpha * coef[k], DTy, 1.0));
+            h.kx[k] = tx;
+            h.ky[k] = ty;
+            if (!j->fused) {  // full E_T for the unfused order
+                void *p;
+                QSIM_TRY(dalloc(j, &p, j->n * sizeof(c128)));
+                c128 *et = (c128 *)p;
+                const size_t ny = d.ny, nxx = d.nx;
+                const double ax = DTx, ay = DTy;
+                const cd a = alpha * coef[k];
+                const c128 ac = mk(a.real(), a.imag());
+                QSIM_TRY(each(j->n, s, [=] __device__(size_t idx) {
+                    double fi = (double)fold_half(idx / ny, nxx), fj = (double)fold_half(idx % ny, ny);
+                    et[idx] = cexp(ac * (fi * fi * ax + fj * fj * ay));
+                }));
+                j->et_full[k] = et;
+            }
+        }
+    } else if (d.boundary == 0) {
+        // f = -I * DT * dt * tt / hbar (src/SplittingMethod1D.cpp:52-56)
+        for (int k = 0; k < nk; ++k) {
+            cd f = cd(-0.0, -1.0) * DTx * dt * coef[k] / d.hbar;
+            c128 *t;
+            QSIM_TRY(build_axis(j, &t, d.nx, f, 1.0, j->fused ? 1.0 / (double)d.nx : 1.0));
+            h.kx[k] = t;
+            j->et_full[k] = t;
+        }
+    } else {
+        // wall: exp(f i^2), i < N, f = -I * DT * dt * tt / hbar, Dp = hbar pi / (dx N)
+        const double Dp = d.hbar * M_PI / (dx * d.nx);
+        const double DT = Dp * Dp / (2 * d.mass);
+        for (int k = 0; k < nk; ++k) {
+            cd f = cd(-0.0, -1.0) * DT * dt * coef[k] / d.hbar;
+            std::vector<c128> hv(d.nx);
+            for (size_t i = 0; i < d.nx; ++i) {
+                cd e = std::exp(f * ((double)i * (double)i));
+                hv[i] = mk(e.real(), e.imag());
+            }
+            void *p;
+            QSIM_TRY(dalloc(j, &p, d.nx * sizeof(c128)));
+            QSIM_HIP(hipMemcpy(p, hv.data(), d.nx * sizeof(c128), hipMemcpyHostToDevice));
+            j->et_full[k] = (c128 *)p;
+        }
+    }
+    if (j->fused) {
+        if (d.precision == 32) {
+            void *p;
+            QSIM_TRY(dalloc(j, &p, j->n * sizeof(c64)));
+            j->work = p;
+        } else {
+            j->work = j->psi;
+        }
+        h.work = j->work;
+        QSIM_TRY(twiddle_table(d.nx, d.precision, &j->tw_x));
+        if (d.dim == 2) QSIM_TRY(twiddle_table(d.ny, d.precision, &j->tw_y));
+    } else {
+        size_t dims[2] = {d.nx, d.ny};
+        size_t tn = j->n;
+        if (d.dim == 1 && d.boundary == 1) {
+            dims[0] = 2 * d.nx;
+            tn = 2 * d.nx;
+        }
+        QSIM_TRY(qsim_hip_fft_plan_create(&j->fwd, d.dim, dims, 0, 64));
+        QSIM_TRY(qsim_hip_fft_plan_create(&j->inv, d.dim, dims, 1, 64));
+        void *p;
+        QSIM_TRY(dalloc(j, &p, tn * sizeof(c128)));
+        j->tmp = (c128 *)p;
+        QSIM_TRY(dalloc(j, &p, tn * sizeof(c128)));
+        j->tmp2 = (c128 *)p;
+    }
+    void *p;
+    QSIM_TRY(dalloc(j, &p, sizeof(SplitDev)));
+    j->dev = (SplitDev *)p;
+    QSIM_HIP(hipMemcpy(j->dev, &h, sizeof(SplitDev), hipMemcpyHostToDevice));
+    QSIM_HIP(hipDeviceSynchronize());
+    return QSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsim_hip_split_create(qsim_split_job **job, const qsim_split_desc *d, const double *V_re,
+                          const double *V_im, qsim_c128 *psi)
+{
+    QSIM_CHECK(job && d, "null argument");
+    *job = nullptr;
+    QSIM_CHECK(d->dim == 1 || d->dim == 2, "dim must be 1 or 2");
+    QSIM_CHECK(d->nx >= 1 && (d->dim == 1 || d->ny >= 1), "empty grid");
+    QSIM_CHECK(d->method == 2 || d->method == 4, "method must be 2 (O2) or 4 (O4)");
+    QSIM_CHECK(d->boundary == 0 || (d->boundary == 1 && d->dim == 1),
+               "unsupported boundary condition!");
+    QSIM_CHECK(d->precision == 64 || d->precision == 32, "precision must be 64 or 32");
+    QSIM_CHECK(d->mass != 0 && d->hbar != 0, "mass and hbar must be non-zero");
+    QSIM_CHECK(d->x1 != d->x0 && (d->dim == 1 || d->y1 != d->y0), "empty domain");
+    QSIM_CHECK(V_re && psi, "null V or psi");
+    qsim_split_job *j = new qsim_split_job();
+    j->d = *d;
+    if (d->dim == 1) j->d.ny = 1;
+    j->n = j->d.nx * j->d.ny;
+    j->psi = reinterpret_cast<c128 *>(psi);
+    int rc = create(j, V_re, V_im);
+    if (rc != QSIM_OK) {
+        destroy(j);
+        return rc;
+    }
+    *job = j;
+    return QSIM_OK;
+}
+
+int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
+{
+    QSIM_CHECK(j, "null job");
+    if (nsteps == 0) return QSIM_OK;
+    hipStream_t s = as_stream(stream);
+    if (j->fused) return fused_steps(j, j->dev, 1, nsteps, s);
+    return unfused_steps(j, nsteps, s);
+}
+
+int qsim_hip_split_steps_timed(qsim_split_job *j, size_t nsteps, double *kernel_ms,
+                               int *kernel_launches, qsim_stream_t stream)
+{
+    QSIM_CHECK(j && kernel_ms && kernel_launches, "null argument");
+    if (!j->fused)
+        return set_error(QSIM_ERR_UNSUPPORTED, "per-kernel timing needs a fused job");
+    for (int k = 0; k < 3; ++k) {
+        kernel_ms[k] = 0;
+        kernel_launches[k] = 0;
+    }
+    if (nsteps == 0) return QSIM_OK;
+    hipStream_t s = as_stream(stream);
+    SweepTimer tm;
+    QSIM_TRY(fused_steps(j, j->dev, 1, nsteps, s, &tm));
+    QSIM_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < tm.kind.size(); ++i) {
+        float ms = 0;
+        QSIM_HIP(hipEventElapsedTime(&ms, tm.ev[2 * i], tm.ev[2 * i + 1]));
+        kernel_ms[tm.kind[i]] += ms;
+        kernel_launches[tm.kind[i]] += 1;
+    }
+    return QSIM_OK;
+}
+
+int qsim_hip_split_load_psi(qsim_split_job *j, qsim_stream_t)
+{
+    // every qsim_hip_split_steps call starts from the caller's psi
+    QSIM_CHECK(j, "null job");
+    return QSIM_OK;
+}
+
+int qsim_hip_split_norm2(qsim_split_job *j, double *out, qsim_stream_t s)
+{
+    QSIM_CHECK(j && out, "null argument");
+    return qsim_hip_norm2(reinterpret_cast<qsim_c128 *>(j->psi), j->n, out, s);
+}
+
+int qsim_hip_split_destroy(qsim_split_job *j)
+{
+    destroy(j);
+    return QSIM_OK;
+}
+
+int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps,
+                                 qsim_stream_t stream)
+{
+    QSIM_CHECK(jobs && njobs >= 0, "null job list");
+    if (njobs == 0 || nsteps == 0) return QSIM_OK;
+    qsim_split_job *j0 = jobs[0];
+    QSIM_CHECK(j0, "null job");
+    bool same = j0->fused;
+    for (int i = 1; i < njobs && same; ++i) {
+        const qsim_split_job *ji = jobs[i];
+        QSIM_CHECK(ji, "null job");
+        same = ji->fused && ji->d.dim == j0->d.dim && ji->d.nx == j0->d.nx &&
+               ji->d.ny == j0->d.ny && ji->d.method == j0->d.method &&
+               ji->d.precision == j0->d.precision;
+    }
+    hipStream_t s = as_stream(stream);
+    if (!same || njobs == 1) {
+        for (int i = 0; i < njobs; ++i) QSIM_TRY(qsim_hip_split_steps(jobs[i], nsteps, stream));
+        return QSIM_OK;
+    }
+    // one launch per sweep for all jobs: gather their descriptors
+    static thread_local SplitDev *table = nullptr;
+    static thread_local int table_cap = 0;
+    if (table_cap < njobs) {
+        if (table) (void)hipFree(table);
+        table = nullptr;
+        table_cap = 0;
+        QSIM_HIP(hipMalloc(&table, sizeof(SplitDev) * njobs));
+        table_cap = njobs;
+    }
+    std::vector<SplitDev> h(njobs);
+    for (int i = 0; i < njobs; ++i) h[i] = jobs[i]->host;
+    QSIM_HIP(hipMemcpyAsync(table, h.data(), sizeof(SplitDev) * njobs, hipMemcpyHostToDevice, s));
+    QSIM_TRY(fused_steps(j0, table, njobs, nsteps, s));
+    // the host vector must outlive the async copy
+    QSIM_HIP(hipStreamSynchronize(s));
+    return QSIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+"""Born-series scattering on the GPU (fused power-of-two passes and the
+unfused path) against the FP64 oracle restatement of src/Perburbation.h.
+
+Bar: complex128 iterates within 1e-12 relative L2; epsilon and cross sections
+to 1e-12 relative; FP32 iterates within the stated tolerance."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+PRE_NAMES = {0: "BornIdentityPreconditioner", 1: "VellekoopPreconditioner",
+             2: "Hao1Preconditioner", 3: "Hao2Preconditioner"}
+
+
+def opts(preconditional, pre, slow, order, precision="fp64"):
+    from qusim_amd.api import Options
+    o = Options().Hip().Order(order).Preconditional(bool(preconditional)).Slow(slow)
+    if preconditional:
+        getattr(o, PRE_NAMES[pre])()
+    if precision != "fp64":
+        o.Precision(precision)
+    return o
+
+
+def pot2d(shape, lo, hi):
+    xs = [lo[i] + (hi[i] - lo[i]) / shape[i] * np.arange(shape[i]) for i in range(2)]
+    X, Y = np.meshgrid(*xs, indexing="ij")
+    return 0.3 * np.exp(-(X ** 2 + Y ** 2) / 4) + 0.05j * np.exp(-(X ** 2 + Y ** 2) / 9)
+
+
+def pot3d(shape, lo, hi):
+    xs = [lo[i] + (hi[i] - lo[i]) / shape[i] * np.arange(shape[i]) for i in range(3)]
+    X, Y, Z = np.meshgrid(*xs, indexing="ij")
+    return (0.2 * np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 4)).astype(np.complex128)
+
+
+CASES_2D = [(0, 0, 1.0), (1, 0, 1.0), (1, 0, 0.6), (1, 1, 0.7), (1, 2, 0.7), (1, 3, 0.5)]
+
+
+@pytest.mark.parametrize("shape", [(128, 128), (96, 80)])
+@pytest.mark.parametrize("pc,pre,slow", CASES_2D)
+def test_born2d_matches_oracle(shape, pc, pre, slow, oracle):
+    from qusim_amd.api import QuPerturbation2D, SolverMethod
+    lo, hi = (-60.0, -50.0), (60.0, 50.0)
+    V = pot2d(shape, lo, hi)
+    E, d = 2.0, (1.0, 0.3)
+    s = QuPerturbation2D().init(V, lo[0], hi[0], shape[0], lo[1], hi[1], shape[1], E, 0.01,
+                                d[0], d[1], SolverMethod.BornSerise, 1.0, 1.0,
+                                opts(pc, pre, slow, 5))
+    s.Compute()
+    s.Compute()
+    got = s.GetPsi()
+    ref, veff, eps, dn = oracle.born(V, lo, hi, E, d, order=5, ncalls=2, preconditional=pc,
+                                     preconditioner=pre, slow=slow, epsilon=0.01)
+    assert rel_l2(got, ref) < 1e-12
+    assert abs(s.GetEpsilon() - eps) <= 1e-12 * abs(eps)
+    assert rel_l2(s.GetVabsb(), veff) < 1e-15
+    assert abs(s.GetDeltaPsiNorm() - dn) < 1e-10 * max(dn, 1e-30)
+    # 2D differential cross section toward 30 degrees
+    psi0 = oracle.plane_wave(shape, lo, hi, E, d)
+    xs = oracle.xsection2d(psi0, ref, veff, lo, hi, E, np.cos(0.5), np.sin(0.5))
+    assert abs(s.ComputeXSection(np.cos(0.5), np.sin(0.5)) - xs) < 1e-11 * xs
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (24, 20, 28), (64, 32, 128)])
+@pytest.mark.parametrize("pc,pre,slow", [(0, 0, 1.0), (1, 1, 0.5), (1, 3, 0.5)])
+def test_born3d_matches_oracle(shape, pc, pre, slow, oracle):
+    from qusim_amd.api import QuPerturbation3D, SolverMethod
+    lo = tuple(-s / 2.0 for s in shape)
+    hi = tuple(s / 2.0 for s in shape)
+    V = pot3d(shape, lo, hi)
+    E, d = 4.0, (0.0, 0.2, 1.0)
+    s = QuPerturbation3D().init(V, lo[0], hi[0], shape[0], lo[1], hi[1], shape[1], lo[2], hi[2],
+                                shape[2], E, 0.01, d[0], d[1], d[2], SolverMethod.BornSerise,
+                                1.0, 1.0, opts(pc, pre, slow, 4))
+    s.Compute()
+    got = s.GetPsi()
+    ref, veff, eps, _ = oracle.born(V, lo, hi, E, d, order=4, preconditional=pc,
+                                    preconditioner=pre, slow=slow, epsilon=0.01, threads=8)
+    assert rel_l2(got, ref) < 1e-12
+    psi0 = oracle.plane_wave(shape, lo, hi, E, d)
+    xs, amp = oracle.xsection3d(psi0, ref, veff, lo, hi, E, 0.3, 0.1, 0.9)
+    assert abs(s.ComputeXSection(0.3, 0.1, 0.9) - xs) < 1e-11 * xs
+
+
+def test_batched_energies_equal_single_runs():
+    """Every energy of a batch gives the same iterate as its own single run."""
+    from qusim_amd.api import QuPerturbation3D, SolverMethod
+    shape, lo, hi = (32, 32, 32), (-16.0,) * 3, (16.0,) * 3
+    V = pot3d(shape, lo, hi)
+    Es = [2.0, 3.5, 5.0]
+    o = opts(1, 1, 0.5, 6)
+    b = QuPerturbation3D().init(V, lo[0], hi[0], 32, lo[1], hi[1], 32, lo[2], hi[2], 32, Es,
+                                0.0, 0, 0, 1, SolverMethod.BornSerise, 1.0, 1.0, o)
+    b.Compute()
+    for k, E in enumerate(Es):
+        s = QuPerturbation3D().init(V, lo[0], hi[0], 32, lo[1], hi[1], 32, lo[2], hi[2], 32, E,
+                                    0.0, 0, 0, 1, SolverMethod.BornSerise, 1.0, 1.0, o)
+        s.Compute()
+        assert np.array_equal(b.GetPsi(k), s.GetPsi())
+
+
+def test_tolerance_stops_and_converges(oracle):
+    from qusim_amd.api import QuPerturbation3D, SolverMethod
+    shape, lo, hi = (32, 32, 32), (-16.0,) * 3, (16.0,) * 3
+    V = pot3d(shape, lo, hi)
+    s = QuPerturbation3D().init(V, lo[0], hi[0], 32, lo[1], hi[1], 32, lo[2], hi[2], 32, 3.0,
+                                0.0, 0, 0, 1, SolverMethod.BornSerise, 1.0, 1.0, opts(1, 1, 0.5, 1))
+    s.Compute(max_iter=400, tol=1e-6)
+    print("iterations to 1e-6:", s.iters[0], "residual", s.residual[0])
+    assert s.iters[0] < 400 and s.residual[0] < 1e-6
+    ref, _, _, _ = oracle.born(V, lo, hi, 3.0, (0, 0, 1), order=s.iters[0], preconditional=1,
+                               preconditioner=1, slow=0.5, threads=8)
+    assert rel_l2(s.GetPsi(), ref) < 1e-12
+
+
+def test_fp32_iterates_error(oracle):
+    from qusim_amd.api import QuPerturbation3D, SolverMethod
+    shape, lo, hi = (64, 64, 64), (-16.0,) * 3, (16.0,) * 3
+    V = pot3d(shape, lo, hi)
+    s = QuPerturbation3D().init(V, lo[0], hi[0], 64, lo[1], hi[1], 64, lo[2], hi[2], 64, 3.0, 0.0,
+                                0, 0, 1, SolverMethod.BornSerise, 1.0, 1.0,
+                                opts(1, 1, 0.5, 60, precision="mixed32"))
+    s.Compute()
+    ref, _, _, _ = oracle.born(V, lo, hi, 3.0, (0, 0, 1), order=60, preconditional=1,
+                               preconditioner=1, slow=0.5, threads=8)
+    err = rel_l2(s.GetPsi(), ref)
+    print("fp32 Born 64^3 x60 rel L2", err)
+    assert err < 1e-5

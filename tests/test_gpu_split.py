@@ -1,0 +1,165 @@
+"""Split-operator evolution on the GPU (fused sweeps and the unfused
+reference-order path) against the FP64 oracle (CpuSeq + KISS restatement).
+
+Bar: complex128 psi within 1e-12 relative L2 (the path's stated tolerance);
+FP32-mixed within the tolerances written beside each case."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def gauss2d(n0, n1, L0=(-30.0, 30.0), L1=(-20.0, 25.0)):
+    x = L0[0] + (L0[1] - L0[0]) / n0 * np.arange(n0)
+    y = L1[0] + (L1[1] - L1[0]) / n1 * np.arange(n1)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    psi = np.exp(0.8j * X - ((X + 5) ** 2 + Y ** 2) / (4 * 9.0))
+    V = 0.5 * np.exp(-(X ** 2 + Y ** 2) / 8) - 0.05j * np.exp(-((X - 3) ** 2 + Y ** 2) / 30)
+    return psi, V, L0, L1
+
+
+def run_1d(psi, V, x0, x1, dt, steps, method, boundary=0, precision="fp64", complex_v=False,
+           batch=1):
+    from qusim_amd.api import BoundaryCondition, Evolver1D, Options, SolverMethod
+    o = Options().Hip().Batch(batch)
+    if precision != "fp64":
+        o.Precision(precision)
+    if complex_v:
+        o.ComplexPotential(True)
+    e = Evolver1D().init(psi, False, dt, False, V, x0, x1, psi.size,
+                         BoundaryCondition.Period if boundary == 0 else BoundaryCondition.InfiniteWall,
+                         SolverMethod.SplittingMethodO2 if method == 2 else SolverMethod.SplittingMethodO4,
+                         1.0, 1.0, o)
+    e.steps(steps // batch)
+    return e
+
+
+def run_2d(psi, V, L0, L1, dt, steps, method, precision="fp64", complex_v=False):
+    from qusim_amd.api import BoundaryCondition, Evolver2D, Options, SolverMethod
+    o = Options().Hip().Batch(steps)
+    if precision != "fp64":
+        o.Precision(precision)
+    if complex_v:
+        o.ComplexPotential(True)
+    e = Evolver2D().init(psi, False, dt, False, V, L0[0], L0[1], psi.shape[0], L1[0], L1[1],
+                         psi.shape[1], BoundaryCondition.Period,
+                         SolverMethod.SplittingMethodO2 if method == 2 else SolverMethod.SplittingMethodO4,
+                         1.0, 1.0, o)
+    e.step()
+    return e
+
+
+def test_c1_tunnelling_matches_oracle(oracle):
+    """C1: N=2048, dt=0.5, 1000 steps, O2 -- psi, norm and P(x>0)."""
+    from qusim_amd.workloads import c1_problem
+    p = c1_problem()
+    e = run_1d(p["psi"], p["V"], p["x0"], p["x1"], p["dt"], p["steps"], 2, batch=50)
+    got = e.GetPsi()
+    ref = oracle.split1d(p["psi"], p["V"], p["x0"], p["x1"], p["dt"], p["steps"], method=2)
+    assert rel_l2(got, ref) < 1e-12
+    dx = 200.0 / 2048
+    assert abs(e.Norm2() - oracle.norm2(ref) * dx) < 1e-12
+    assert abs(e.Norm2(0.0, 100.0) - oracle.norm2_range_1d(ref, -100, 100, 0, 100)) < 1e-12
+
+
+@pytest.mark.parametrize("n,method", [(256, 4), (4096, 2), (64, 2)])
+def test_1d_fused_matches_oracle(n, method, oracle):
+    x = -50 + 100.0 / n * np.arange(n)
+    psi = np.exp(1.3j * x - (x + 10) ** 2 / 20)
+    V = 0.4 * np.exp(-x ** 2 / 3)
+    got = run_1d(psi, V, -50.0, 50.0, 0.05, 120, method).GetPsi()
+    ref = oracle.split1d(psi, V, -50.0, 50.0, 0.05, 120, method=method)
+    assert rel_l2(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n,boundary,method", [(1000, 0, 2), (1000, 1, 4), (512, 1, 2)])
+def test_1d_unfused_paths_match_oracle(n, boundary, method, oracle):
+    """non-power-of-two length (generic FFT) and the InfiniteWall sine path"""
+    x = -30 + 60.0 / n * np.arange(n)
+    psi = np.exp(1.0j * x - (x + 5) ** 2 / 10)
+    V = 0.3 * np.exp(-x ** 2 / 2)
+    got = run_1d(psi, V, -30.0, 30.0, 0.02, 60, method, boundary=boundary).GetPsi()
+    ref = oracle.split1d(psi, V, -30.0, 30.0, 0.02, 60, method=method, boundary=boundary)
+    assert rel_l2(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(128, 64), (64, 256), (512, 512)])
+@pytest.mark.parametrize("method", [2, 4])
+def test_2d_fused_matches_oracle(shape, method, oracle):
+    psi, V, L0, L1 = gauss2d(*shape)
+    got = run_2d(psi, V, L0, L1, 0.05, 40, method).GetPsi()
+    ref = oracle.split2d(psi, V, L0[0], L0[1], L1[0], L1[1], 0.05, 40, method=method)
+    assert rel_l2(got, ref) < 1e-12
+
+
+def test_2d_complex_potential_and_complex_dt(oracle):
+    psi, V, L0, L1 = gauss2d(128, 128)
+    dt = 0.05 - 0.01j
+    got = run_2d(psi, V, L0, L1, dt, 30, 2, complex_v=True).GetPsi()
+    ref = oracle.split2d(psi, V, L0[0], L0[1], L1[0], L1[1], dt, 30, method=2, complex_v=True)
+    assert rel_l2(got, ref) < 1e-12
+
+
+def test_2d_unfused_matches_oracle(oracle):
+    psi, V, L0, L1 = gauss2d(96, 80)
+    for m in (2, 4):
+        got = run_2d(psi, V, L0, L1, 0.05, 25, m).GetPsi()
+        ref = oracle.split2d(psi, V, L0[0], L0[1], L1[0], L1[1], 0.05, 25, method=m)
+        assert rel_l2(got, ref) < 1e-12
+
+
+def test_c2_shape_prefix_matches_oracle(oracle):
+    """C2 at full size (1024^2, dt=0.2, complex V) for 20 steps."""
+    from qusim_amd.workloads import c2_problem
+    p = c2_problem()
+    L = (p["x0"], p["x1"])
+    got = run_2d(p["psi"], p["V"], L, L, p["dt"], 20, 2, complex_v=True).GetPsi()
+    ref = oracle.split2d(p["psi"], p["V"], L[0], L[1], L[0], L[1], p["dt"], 20, method=2,
+                         complex_v=True, threads=16)
+    assert rel_l2(got, ref) < 1e-12
+
+
+def test_fp32_mixed_error_bounds(oracle):
+    psi, V, L0, L1 = gauss2d(256, 256)
+    got = run_2d(psi, V, L0, L1, 0.05, 200, 2, precision="mixed32", complex_v=True).GetPsi()
+    ref = oracle.split2d(psi, V, L0[0], L0[1], L1[0], L1[1], 0.05, 200, method=2, complex_v=True)
+    err = rel_l2(got, ref)
+    print("fp32-mixed 256^2 x200 rel L2", err)
+    assert err < 1e-4
+    x = -50 + 100.0 / 1024 * np.arange(1024)
+    p1 = np.exp(1.3j * x - (x + 10) ** 2 / 20)
+    V1 = 0.4 * np.exp(-x ** 2 / 3)
+    g1 = run_1d(p1, V1, -50.0, 50.0, 0.05, 400, 2, precision="mixed32").GetPsi()
+    r1 = oracle.split1d(p1, V1, -50.0, 50.0, 0.05, 400, method=2)
+    err1 = rel_l2(g1, r1)
+    print("fp32-mixed 1D 1024 x400 rel L2", err1)
+    assert err1 < 1e-4
+
+
+def test_batched_equals_alone():
+    """Each job's result is bitwise the same alone and inside a batched launch."""
+    from qusim_amd.api import advance_batched
+    outs_alone, evs = [], []
+    for k in range(3):
+        psi, V, L0, L1 = gauss2d(128, 128)
+        V = V * (1 + 0.3 * k)
+        a = run_2d(psi, V, L0, L1, 0.05 * (k + 1), 10, 2, complex_v=True)
+        outs_alone.append(a.GetPsi())
+        b = run_2d(psi, V, L0, L1, 0.05 * (k + 1), 0, 2, complex_v=True)
+        evs.append(b)
+    advance_batched(evs, 10)
+    for a, b in zip(outs_alone, evs):
+        assert np.array_equal(a, b.GetPsi())
+
+
+def test_norm_conserved_for_real_potential():
+    psi, V, L0, L1 = gauss2d(256, 256)
+    e = run_2d(psi, V.real, L0, L1, 0.1, 0, 2)
+    n0 = e.Norm2()
+    e.steps(1)
+    for _ in range(5):
+        from qusim_amd._native import call
+        call("qsim_hip_split_steps", e._job, 100, None)
+    assert abs(e.Norm2() / n0 - 1) < 1e-12
