@@ -1,0 +1,279 @@
+"""CPU pins of the oracle (oracle/qsim_oracle.c).  Each check is independent of
+the oracle's own formulas: golden vectors of the compiled reference, closed
+forms, invariants, brute force, and the reference's published known answers.
+A planted mistake in the oracle (a sign, an index, a dropped term) fails one."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "ref_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def G():
+    return np.load(GOLD, allow_pickle=False)
+
+
+def ramp(shape):
+    return np.arange(int(np.prod(shape)), dtype=np.float64).reshape(shape).astype(np.complex128)
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 12, 16, 17, 64, 100, 243, 1234])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fft_vs_bruteforce(oracle, n, inverse):
+    x = np.random.default_rng(n).standard_normal(n) + 1j * np.random.default_rng(n + 1).standard_normal(n)
+    assert rel_l2(oracle.fft(x, inverse), oracle.dft_bruteforce(x, inverse)) < 2e-15
+
+
+def test_fft_closed_forms(oracle):
+    n = 96
+    d = np.zeros(n, np.complex128)
+    d[0] = 1
+    assert np.allclose(oracle.fft(d), np.ones(n), atol=1e-15)          # delta -> constant
+    k0 = 5
+    e = np.exp(2j * np.pi * k0 * np.arange(n) / n)                      # exponential -> delta
+    want = np.zeros(n, np.complex128)
+    want[k0] = n
+    assert np.max(np.abs(oracle.fft(e) - want)) < 1e-12
+    x = np.random.default_rng(0).standard_normal((12, 23, 34)).astype(np.complex128)
+    assert rel_l2(oracle.fft(oracle.fft(x), True), x * x.size) < 1e-14  # round trip
+    y = oracle.fft(x)
+    assert abs(np.sum(np.abs(y) ** 2) / x.size - np.sum(np.abs(x) ** 2)) < 1e-10  # Parseval
+
+
+@pytest.mark.parametrize("shape", [(123, 234), (12, 23, 34), (64, 48)])
+def test_fft_multi_d_vs_numpy(oracle, shape):
+    x = np.random.default_rng(3).standard_normal(shape) + 0j
+    assert rel_l2(oracle.fft(x), np.fft.fftn(x)) < 1e-14
+    assert rel_l2(oracle.fft(x, True), np.fft.ifftn(x) * x.size) < 1e-14
+
+
+def test_fft_golden(oracle, G):
+    for name, shape in (("fft1234", (1234,)), ("fft123x234", (123, 234)),
+                        ("fft12x23x34", (12, 23, 34)), ("fft64x48", (64, 48))):
+        x = ramp(shape)
+        f, i = oracle.fft(x), oracle.fft(x, True)
+        if name + "_idx" in G:
+            idx = G[name + "_idx"]
+            assert rel_l2(f.ravel()[idx], G[name + "_fwd_sample"]) < 1e-14, name
+            assert rel_l2(i.ravel()[idx], G[name + "_inv_sample"]) < 1e-14, name
+            s = G[name + "_sums"]
+            assert abs(np.sum(np.abs(f) ** 2) - s[1].real) < 1e-13 * s[1].real, name
+        else:
+            assert rel_l2(f, G[name + "_fwd"]) < 1e-14, name
+            assert rel_l2(i, G[name + "_inv"]) < 1e-14, name
+
+
+# -------------------------------------------------------------- split step
+def test_split_golden(oracle, G):
+    psi = G["c1_psi0"]
+    x = -100 + 200.0 / 2048 * np.arange(2048)
+    V = 0.6 * np.exp(-x ** 2 / 2)
+    assert rel_l2(oracle.split1d(psi, V, -100.0, 100.0, 0.5, 1000, method=2), G["c1_o2_1000"]) < 1e-12
+    assert rel_l2(oracle.split1d(psi, V, -100.0, 100.0, 0.5, 200, method=4), G["c1_o4_200"]) < 1e-12
+    assert rel_l2(oracle.split1d(psi, V, -100.0, 100.0, 0.5, 200, method=2, boundary=1),
+                  G["c1_wall_o2_200"]) < 1e-12
+    assert rel_l2(oracle.split1d(psi, V, -100.0, 100.0, 0.5 - 0.05j, 100, method=2),
+                  G["c1_cdt_o2_100"]) < 1e-12
+    p2, V2 = G["s2_psi0"], G["s2_V"]
+    assert rel_l2(oracle.split2d(p2, V2, -30.0, 30.0, -20.0, 25.0, 0.1, 50, method=2),
+                  G["s2_o2_50"]) < 1e-12
+    assert rel_l2(oracle.split2d(p2, V2, -30.0, 30.0, -20.0, 25.0, 0.1, 20, method=4),
+                  G["s2_o4_20"]) < 1e-12
+
+
+def _packet(n, L=50.0, k=1.3, c=-10.0):
+    x = -L + 2 * L / n * np.arange(n)
+    return x, np.exp(1j * k * x - (x - c) ** 2 / 20)
+
+
+@pytest.mark.parametrize("n", [512, 243])
+def test_free_evolution_closed_form(oracle, n):
+    """V = 0: the split step is exactly exp(-i k^2 T/2) in k space (odd n pins
+    the frequency folding of the Nyquist neighbourhood)."""
+    T, steps = 3.0, 60
+    x, psi = _packet(n)
+    k = 2 * np.pi * np.fft.fftfreq(n, d=100.0 / n)
+    want = np.fft.ifft(np.exp(-0.5j * k * k * T) * np.fft.fft(psi))
+    for m in (2, 4):
+        got = oracle.split1d(psi, np.zeros(n), -50.0, 50.0, T / steps, steps, method=m)
+        assert rel_l2(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(128, 96), (75, 121)])
+def test_free_evolution_closed_form_2d(oracle, shape):
+    T, steps = 3.0, 60
+    L = (-40.0, 40.0)
+    xs = [L[0] + (L[1] - L[0]) / s * np.arange(s) for s in shape]
+    X, Y = np.meshgrid(*xs, indexing="ij")
+    p2 = np.exp(1j * X - (X ** 2 + Y ** 2) / 20)
+    kx = 2 * np.pi * np.fft.fftfreq(shape[0], d=(L[1] - L[0]) / shape[0])
+    ky = 2 * np.pi * np.fft.fftfreq(shape[1], d=(L[1] - L[0]) / shape[1])
+    KX, KY = np.meshgrid(kx, ky, indexing="ij")
+    want2 = np.fft.ifft2(np.exp(-0.5j * (KX ** 2 + KY ** 2) * T) * np.fft.fft2(p2))
+    for m in (2, 4):
+        got2 = oracle.split2d(p2, np.zeros(shape), L[0], L[1], L[0], L[1], T / steps, steps,
+                              method=m)
+        assert rel_l2(got2, want2) < 1e-12
+
+
+def test_norm_conservation_and_time_reversal(oracle):
+    n = 1024
+    x, psi = _packet(n)
+    V = 0.5 * np.exp(-x ** 2)
+    fwd = oracle.split1d(psi, V, -50.0, 50.0, 0.05, 400, method=2)
+    assert abs(oracle.norm2(fwd) / oracle.norm2(psi) - 1) < 1e-13
+    back = oracle.split1d(fwd, V, -50.0, 50.0, -0.05, 400, method=2)
+    assert rel_l2(back, psi) < 1e-12
+
+
+def test_convergence_orders(oracle):
+    """error(dt) ~ dt^2 for O2 and ~ dt^4 for O4 against a fine reference"""
+    n, T = 512, 2.0
+    x, psi = _packet(n, k=1.0, c=-3.0)
+    V = 1.0 * np.exp(-x ** 2)
+    ref = oracle.split1d(psi, V, -50.0, 50.0, T / 2048, 2048, method=4)
+    for m, order in ((2, 2), (4, 4)):
+        e1 = rel_l2(oracle.split1d(psi, V, -50.0, 50.0, T / 32, 32, method=m), ref)
+        e2 = rel_l2(oracle.split1d(psi, V, -50.0, 50.0, T / 64, 64, method=m), ref)
+        assert abs(math.log2(e1 / e2) - order) < 0.3, (m, e1, e2)
+
+
+def test_tunnelling_known_answer(oracle):
+    """applications/QuEvol1DPrecision/QuSimPrecision.cpp:82: P(x > 0) at T = 64
+    for gauss(x,-20,5) e^{ix}, V = exp(-x^2) on [-100,100) = 0.15937744085."""
+    n, x0, x1, dt = 2000, -100.0, 100.0, 0.04
+    dx = (x1 - x0) / n
+    x = dx * np.arange(n) + x0
+    g = np.exp(-(x + 20) ** 2 / 50.0) / (5 * math.sqrt(2 * math.pi))  # Cal gauss = normal pdf
+    psi = g * np.exp(1j * x)
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * dx)
+    p = oracle.split1d(psi, np.exp(-x * x), x0, x1, dt, 1600, method=4)
+    assert abs(oracle.norm2_range_1d(p, x0, x1, 0.0, 100.0) - 0.15937744085) < 5e-8
+
+
+def test_free_packet_velocity(oracle):
+    """applications/QuEvol1DPrecision/QuSimPrecision.cpp:210-218: (Xavg(T)-Xavg(0))/T = 1"""
+    n, x0, x1 = 4000, -100.0, 100.0
+    dx = (x1 - x0) / n
+    x = dx * np.arange(n) + x0
+    psi = np.exp(-x ** 2 / 50.0) * np.exp(1j * x)
+    T, steps = 20.0, 400
+    p = oracle.split1d(psi, np.zeros(n), x0, x1, T / steps, steps, method=2)
+
+    def xavg(q):
+        w = np.abs(q) ** 2
+        return x0 + dx * np.sum(w * np.arange(n)) / np.sum(w)
+
+    assert abs((xavg(p) - xavg(psi)) / T - 1) < 1e-6
+
+
+# ------------------------------------------------------------------ Born
+def test_born_golden(oracle, G):
+    lo2, hi2 = (-60.0, -50.0), (60.0, 50.0)
+    for tag, pc, pre, slow in (("naive", 0, 0, 1.0), ("ident", 1, 0, 0.6), ("vell", 1, 1, 0.7),
+                               ("hao1", 1, 2, 0.7), ("hao2", 1, 3, 0.5)):
+        dp, veff, eps, dn = oracle.born(G["b2_V"], lo2, hi2, 2.0, (1.0, 0.3), order=3, ncalls=2,
+                                        preconditional=pc, preconditioner=pre, slow=slow,
+                                        epsilon=0.01)
+        assert rel_l2(dp, G["b2_%s_dpsi" % tag]) < 1e-13, tag
+        eg, dg = G["b2_%s_scalars" % tag]
+        assert abs(eps - eg) <= 1e-13 * abs(eg) and abs(dn - dg) <= 1e-11 * abs(dg), tag
+        if tag == "vell":
+            assert rel_l2(veff, G["b2_vell_veff"]) == 0.0
+    lo3, hi3 = (-6.0, -5.0, -7.0), (6.0, 5.0, 7.0)
+    for tag, pc, pre, slow in (("naive", 0, 0, 1.0), ("vell", 1, 1, 0.5), ("hao2", 1, 3, 0.5)):
+        dp, veff, eps, dn = oracle.born(G["b3_V"], lo3, hi3, 4.0, (0.0, 0.2, 1.0), order=3,
+                                        preconditional=pc, preconditioner=pre, slow=slow,
+                                        epsilon=0.01)
+        assert rel_l2(dp, G["b3_%s_dpsi" % tag]) < 1e-13, tag
+        if tag == "vell":
+            psi0 = oracle.plane_wave(G["b3_V"].shape, lo3, hi3, 4.0, (0.0, 0.2, 1.0))
+            xs, _ = oracle.xsection3d(psi0, dp, veff, lo3, hi3, 4.0, 0.3, 0.1, 0.9)
+            assert abs(xs - G["b3_vell_xsec"][0]) < 1e-12 * xs
+
+
+def test_first_born_closed_form_3d(oracle):
+    """applications/QuScattering3DPrecision/QuScattering3DPrecision.cpp:11-21:
+    V = v0 exp(-a r^2) gives dsigma/dOmega = m^2/(4 pi^2 hbar^4) [v0 (pi/a)^1.5 e^{-q^2/4a}]^2."""
+    n, L, v0, a, p = 48, 8.0, 0.01, 1.0, 1.0
+    g = -L + 2 * L / n * np.arange(n)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    V = (v0 * np.exp(-a * (X ** 2 + Y ** 2 + Z ** 2))).astype(np.complex128)
+    E = p * p / 2
+    psi0 = oracle.plane_wave(V.shape, (-L,) * 3, (L,) * 3, E, (0, 0, 1))
+    for th in (0.0, 0.7, 2.0):
+        c = (math.sin(th) * math.cos(0.1234), math.sin(th) * math.sin(0.1234), math.cos(th))
+        got, _ = oracle.xsection3d(psi0, np.zeros_like(V), V, (-L,) * 3, (L,) * 3, E, *c)
+        q2 = (c[0] ** 2 + c[1] ** 2 + (c[2] - 1) ** 2) * p * p
+        mat = v0 * (math.pi / a) ** 1.5 * math.exp(-q2 / (4 * a))
+        want = mat * mat / (2 * math.pi * 2 * math.pi)
+        assert abs(got - want) < 1e-10 * want
+
+
+def test_first_born_closed_form_2d(oracle):
+    """applications/QuScattering2DPrecision/QuScattering2DPrecision.cpp:5-14."""
+    n, L, v0, a, p = 256, 20.0, 0.001, 1.0, 1.0
+    g = -L + 2 * L / n * np.arange(n)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    V = (v0 * np.exp(-a * (X ** 2 + Y ** 2))).astype(np.complex128)
+    E = 0.5
+    psi0 = oracle.plane_wave(V.shape, (-L, -L), (L, L), E, (1, 0))
+    for th in (0.3, 1.5, 3.0):
+        got = oracle.xsection2d(psi0, np.zeros_like(V), V, (-L, -L), (L, L), E, math.cos(th),
+                                math.sin(th))
+        kx, ky = (1 - math.cos(th)) * p, math.sin(th) * p
+        mat = v0 * math.pi * math.exp(-(kx * kx + ky * ky) / (4 * a)) / a
+        want = mat * mat / (2 * math.pi * p)
+        assert abs(got - want) < 1e-10 * want
+
+
+def test_born_fixed_point_is_lippmann_schwinger(oracle):
+    """The converged naive series solves (E - T_k + i eps) dpsi_k = FFT(V (psi0 + dpsi))."""
+    n, L, E, eps = 24, 12.0, 2.0, 0.05
+    g = -L + 2 * L / n * np.arange(n)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    V = (0.05 * np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 3)).astype(np.complex128)
+    dp, _, _, _ = oracle.born(V, (-L,) * 3, (L,) * 3, E, (0, 0, 1), order=80,
+                              preconditional=0, epsilon=eps)
+    psi0 = oracle.plane_wave(V.shape, (-L,) * 3, (L,) * 3, E, (0, 0, 1))
+    k = 2 * np.pi * np.fft.fftfreq(n, d=2 * L / n)
+    KX, KY, KZ = np.meshgrid(k, k, k, indexing="ij")
+    lhs = (E - (KX ** 2 + KY ** 2 + KZ ** 2) / 2 + 1j * eps) * np.fft.fftn(dp)
+    rhs = np.fft.fftn(V * (psi0 + dp))
+    assert rel_l2(lhs, rhs) < 1e-10
+
+
+def test_epsilon_rule(oracle):
+    """GetEpsilon (src/Perburbation.h:329-344): eps = max(eps_user, max|V_eff|) for
+    Vellekoop; 0.5 |V|^2 / (-sum Im V) for Identity."""
+    n = 16
+    lo, hi = (-8.0,) * 3, (8.0,) * 3
+    V = np.full((n, n, n), 0.1 + 0.0j)
+    _, veff, eps, _ = oracle.born(V, lo, hi, 4.0, (0, 0, 1), order=1, preconditional=1,
+                                  preconditioner=1, epsilon=0.0)
+    assert eps == pytest.approx(np.max(np.abs(veff)), rel=1e-15)
+    _, veff0, eps0, _ = oracle.born(V, lo, hi, 4.0, (0, 0, 1), order=1, preconditional=1,
+                                    preconditioner=0, epsilon=0.0)
+    assert eps0 == pytest.approx(0.5 * np.sum(np.abs(veff0) ** 2) / -np.sum(veff0.imag), rel=1e-13)
+
+
+def test_compiled_reference_agrees_when_present(oracle):
+    """With the reference tree present, the oracle equals the compiled reference
+    on fresh random inputs too (not only on the committed vectors)."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((20, 30)) + 1j * rng.standard_normal((20, 30))
+    assert rel_l2(oracle.fft(x), oracle.ref_fft(x)) < 1e-14
+    n = 300
+    xs, psi = _packet(n)
+    V = rng.uniform(0, 1, n) * 0.3
+    assert rel_l2(oracle.split1d(psi, V, -50.0, 50.0, 0.07, 120, method=4),
+                  oracle.ref_split1d(psi, V, -50.0, 50.0, 0.07, 120, method=4)) < 1e-12
