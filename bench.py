@@ -54,6 +54,17 @@ def make_job(N, prob, precision):
     return p.value, (psi, vre, vim)
 
 
+def autotune(job, reps=20):
+    """pick the fastest launch variants of the row / column sweeps"""
+    from qusim_amd import _native as Nn
+    vr, vc = ctypes.c_int(), ctypes.c_int()
+    ru, cu = ctypes.c_double(), ctypes.c_double()
+    Nn.call("qsim_hip_split_autotune", job, reps, ctypes.byref(vr), ctypes.byref(vc),
+            ctypes.byref(ru), ctypes.byref(cu), None)
+    return {"row_variant": vr.value, "col_variant": vc.value, "row_us": ru.value,
+            "col_us": cu.value}
+
+
 def time_steps(job, steps, warmup):
     from qusim_amd import _native as Nn
     if warmup:
@@ -119,6 +130,8 @@ def fp32_section(steps_err, steps_rate):
     n = prob["n"]
     j64, keep64 = make_job(Nn, prob, 64)
     j32, keep32 = make_job(Nn, prob, 32)
+    autotune(j64)
+    autotune(j32)
     wall, gpu = time_steps(j32, steps_rate, 2)
     rate32 = steps_rate * n * n / gpu
     w64, g64 = time_steps(j64, steps_rate, 2)
@@ -172,6 +185,8 @@ def main():
     n = prob["n"]
     npts = n * n
     job, keep = make_job(Nn, prob, 64)
+    tuned = autotune(job)
+    log("autotuned variants: %s" % tuned)
     log("timing %d steps of 1024^2 (warmup %d)" % (args.steps, args.warmup))
     wall, gpu = time_steps(job, args.steps, args.warmup)
     value = args.steps * npts / wall
@@ -197,7 +212,7 @@ def main():
                                "[-150,150)^2, dt=0.2, complex128, 16 seeded Gaussian bumps "
                                "+ imaginary absorbing layer",
                    "grid": [n, n], "precision": "complex128", "method": "Strang O2",
-                   "device": info["name"]},
+                   "device": info["name"], "launch_variants": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(dom),

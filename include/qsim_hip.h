@@ -224,6 +224,17 @@ int qsim_hip_split_steps(qsim_split_job *job, size_t nsteps, qsim_stream_t s);
  * sweeps, 2 the persistent 1D kernel.  Synchronous.  Fused jobs only. */
 int qsim_hip_split_steps_timed(qsim_split_job *job, size_t nsteps, double *kernel_ms,
                                int *kernel_launches, qsim_stream_t s);
+/* 2D fused jobs replay each step count's sweep sequence from a captured HIP
+ * graph (off by default: on this stack a replayed graph was measured slower
+ * than direct launches); enable = 1 turns it on. */
+int qsim_hip_split_use_graph(qsim_split_job *job, int enable);
+/* Launch variants of the 2D sweep kernels (radix set, lines per workgroup):
+ * 0..5 for axis lengths 1024 and 2048, only 0 otherwise. */
+int qsim_hip_split_set_variants(qsim_split_job *job, int row_variant, int col_variant);
+/* Autotuner: times every row / column variant (reps launches each, on a copy
+ * of the state), keeps the fastest on the job and reports it.  Synchronous. */
+int qsim_hip_split_autotune(qsim_split_job *job, int reps, int *row_variant, int *col_variant,
+                            double *row_us, double *col_us, qsim_stream_t s);
 /* Re-read psi after the caller changed it (e.g. renormalised it). */
 int qsim_hip_split_load_psi(qsim_split_job *job, qsim_stream_t s);
 /* sum |psi|^2 over the job's working state (compensated FP64), synchronous */

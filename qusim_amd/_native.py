@@ -126,6 +126,10 @@ _SIGS = {
     "qsim_hip_split_create": [ctypes.POINTER(_VP), ctypes.POINTER(SplitDesc), _VP, _VP, _VP],
     "qsim_hip_split_steps": [_VP, _SZ, _VP],
     "qsim_hip_split_steps_timed": [_VP, _SZ, ctypes.POINTER(_D), ctypes.POINTER(_I), _VP],
+    "qsim_hip_split_set_variants": [_VP, _I, _I],
+    "qsim_hip_split_use_graph": [_VP, _I],
+    "qsim_hip_split_autotune": [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                ctypes.POINTER(_D), ctypes.POINTER(_D), _VP],
     "qsim_hip_split_load_psi": [_VP, _VP],
     "qsim_hip_split_norm2": [_VP, ctypes.POINTER(_D), _VP],
     "qsim_hip_split_destroy": [_VP],

@@ -5,47 +5,77 @@
 
 namespace qsim {
 
-// Radix plan of the register-resident power-of-two line FFT, per size and
-// element type.  N/E threads per line, E = largest radix.
-template <typename T, int N>
+// Radix plan of the register-resident power-of-two line FFT, per size,
+// element type and launch variant V (variant 0 is the default; the others
+// are the candidates the split-step autotuner times).  N/E threads per line,
+// E = largest radix.  ROW_L / COL_L: lines per workgroup of the row
+// (contiguous) and column (strided) kernels; 0 = derive from LDS.
+template <typename T, int N, int V = 0>
 struct P2Cfg;
-#define QSIM_P2CFG(TYPE, N_, A, B, C_)                 \
-    template <>                                        \
-    struct P2Cfg<TYPE, N_> {                           \
-        using Line = Pow2Line<TYPE, N_, A, B, C_>;     \
+#define QSIM_P2CFG(TYPE, N_, V_, A, B, C_, RL, CL)      \
+    template <>                                         \
+    struct P2Cfg<TYPE, N_, V_> {                        \
+        using Line = Pow2Line<TYPE, N_, A, B, C_>;      \
+        static constexpr int ROW_L = RL, COL_L = CL;    \
     };
-QSIM_P2CFG(double, 16, 4, 4, 1)
-QSIM_P2CFG(double, 32, 8, 4, 1)
-QSIM_P2CFG(double, 64, 8, 8, 1)
-QSIM_P2CFG(double, 128, 16, 8, 1)
-QSIM_P2CFG(double, 256, 16, 16, 1)
-QSIM_P2CFG(double, 512, 8, 8, 8)
-QSIM_P2CFG(double, 1024, 16, 16, 4)
-QSIM_P2CFG(double, 2048, 16, 16, 8)
-QSIM_P2CFG(double, 4096, 16, 16, 16)
-QSIM_P2CFG(float, 16, 4, 4, 1)
-QSIM_P2CFG(float, 32, 8, 4, 1)
-QSIM_P2CFG(float, 64, 8, 8, 1)
-QSIM_P2CFG(float, 128, 16, 8, 1)
-QSIM_P2CFG(float, 256, 16, 16, 1)
-QSIM_P2CFG(float, 512, 16, 32, 1)
-QSIM_P2CFG(float, 1024, 32, 32, 1)
-QSIM_P2CFG(float, 2048, 16, 16, 8)
-QSIM_P2CFG(float, 4096, 16, 16, 16)
-QSIM_P2CFG(float, 8192, 16, 16, 32)
+QSIM_P2CFG(double, 16, 0, 4, 4, 1, 0, 0)
+QSIM_P2CFG(double, 32, 0, 8, 4, 1, 0, 0)
+QSIM_P2CFG(double, 64, 0, 8, 8, 1, 0, 0)
+QSIM_P2CFG(double, 128, 0, 16, 8, 1, 0, 0)
+QSIM_P2CFG(double, 256, 0, 16, 16, 1, 0, 0)
+QSIM_P2CFG(double, 512, 0, 8, 8, 8, 0, 0)
+QSIM_P2CFG(double, 1024, 0, 16, 16, 4, 1, 4)
+QSIM_P2CFG(double, 1024, 1, 16, 16, 4, 4, 4)
+QSIM_P2CFG(double, 1024, 2, 32, 32, 1, 2, 4)
+QSIM_P2CFG(double, 1024, 3, 32, 32, 1, 2, 8)
+QSIM_P2CFG(double, 1024, 4, 8, 8, 16, 1, 4)
+QSIM_P2CFG(double, 1024, 5, 16, 16, 4, 2, 2)
+QSIM_P2CFG(double, 2048, 0, 16, 16, 8, 1, 4)
+QSIM_P2CFG(double, 2048, 1, 16, 16, 8, 2, 2)
+QSIM_P2CFG(double, 2048, 2, 32, 32, 2, 1, 4)
+QSIM_P2CFG(double, 2048, 3, 8, 16, 16, 1, 4)
+QSIM_P2CFG(double, 2048, 4, 16, 16, 8, 1, 8)
+QSIM_P2CFG(double, 2048, 5, 16, 16, 8, 4, 4)
+QSIM_P2CFG(double, 4096, 0, 16, 16, 16, 0, 0)
+QSIM_P2CFG(float, 16, 0, 4, 4, 1, 0, 0)
+QSIM_P2CFG(float, 32, 0, 8, 4, 1, 0, 0)
+QSIM_P2CFG(float, 64, 0, 8, 8, 1, 0, 0)
+QSIM_P2CFG(float, 128, 0, 16, 8, 1, 0, 0)
+QSIM_P2CFG(float, 256, 0, 16, 16, 1, 0, 0)
+QSIM_P2CFG(float, 512, 0, 16, 32, 1, 0, 0)
+QSIM_P2CFG(float, 1024, 0, 32, 32, 1, 2, 8)
+QSIM_P2CFG(float, 1024, 1, 32, 32, 1, 4, 16)
+QSIM_P2CFG(float, 1024, 2, 16, 16, 4, 1, 8)
+QSIM_P2CFG(float, 1024, 3, 16, 16, 4, 2, 16)
+QSIM_P2CFG(float, 1024, 4, 32, 32, 1, 2, 4)
+QSIM_P2CFG(float, 1024, 5, 16, 16, 4, 4, 4)
+QSIM_P2CFG(float, 2048, 0, 16, 16, 8, 1, 8)
+QSIM_P2CFG(float, 2048, 1, 32, 64, 1, 1, 8)
+QSIM_P2CFG(float, 2048, 2, 64, 32, 1, 1, 8)
+QSIM_P2CFG(float, 2048, 3, 16, 16, 8, 2, 16)
+QSIM_P2CFG(float, 2048, 4, 32, 32, 2, 1, 8)
+QSIM_P2CFG(float, 2048, 5, 16, 16, 8, 1, 4)
+QSIM_P2CFG(float, 4096, 0, 16, 16, 16, 0, 0)
+QSIM_P2CFG(float, 8192, 0, 16, 16, 32, 0, 0)
 #undef QSIM_P2CFG
 
+__host__ __device__ constexpr int pow2floor(int x) { return x <= 1 ? 1 : 2 * pow2floor(x / 2); }
+
 // Lines per workgroup: rows (position-fast) fill ~256 threads; strided axes
-// (line-fast) take 128 B of adjacent lines per row of the tile.
-template <typename T, int N, bool LF>
+// (line-fast) take 128 B of adjacent lines per row of the tile; both kept
+// within 96 KiB of LDS and a power of two, unless the variant fixes them.
+template <typename T, int N, bool LF, int V = 0>
 struct P2Launch {
-    using Line = typename P2Cfg<T, N>::Line;
+    using Cfg = P2Cfg<T, N, V>;
+    using Line = typename Cfg::Line;
     static constexpr int LF_LINES = 128 / (int)sizeof(cx<T>);
     static constexpr int ROWS = cmax(1, 256 / Line::TPL);
     static constexpr int L_WANT = LF ? LF_LINES : ROWS;
-    // keep the LDS tile within 96 KiB
     static constexpr int L_LDS = cmax(1, (96 * 1024) / (Line::LS * (int)sizeof(cx<T>)));
-    static constexpr int L = cmin(L_WANT, L_LDS);
+    static constexpr int L_FIXED = LF ? Cfg::COL_L : Cfg::ROW_L;
+    // never more lines than the 160 KiB of LDS one workgroup can hold
+    static constexpr int L_CAP = pow2floor(cmax(1, (160 * 1024) / (Line::LS * (int)sizeof(cx<T>))));
+    static constexpr int L = cmin(L_CAP, L_FIXED > 0 ? L_FIXED : pow2floor(cmin(L_WANT, L_LDS)));
     static constexpr int THREADS = L * Line::TPL;
     static constexpr size_t LDS_BYTES = (size_t)L * Line::LS * sizeof(cx<T>);
 };
@@ -87,6 +117,15 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
             for (int r = 0; r < R1; ++r)
                 out[base + (size_t)P::template in_pos<R1>(t, u, r) * S] = v[u * R1 + r] * scale;
     }
+}
+
+// Workgroups are dealt to the 8 XCDs round robin (workgroup b on XCD b % 8).
+// For strided line tiles, give each XCD a contiguous range of tiles so the
+// 128 B rows neighbouring tiles share stay in one L2.
+__device__ __forceinline__ size_t xcd_tile(size_t b, size_t nb)
+{
+    if (nb % 8) return b;
+    return (b % 8) * (nb / 8) + b / 8;
 }
 
 // ---------------------------------------------------------------------------

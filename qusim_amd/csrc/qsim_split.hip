@@ -21,6 +21,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "qsim_fft_kernels.h"
@@ -57,12 +58,12 @@ __device__ __forceinline__ cx<TO> conv(cx<TI> a)
 }
 
 // ---------------------------------------------------------------- 2D sweeps
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((P2Launch<T, N, false>::THREADS))
+template <typename T, int N, int MODE, int V>
+__global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
     k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
                 const cx<T> *__restrict__ tw)
 {
-    using LP = P2Launch<T, N, false>;
+    using LP = P2Launch<T, N, false, V>;
     using P = typename LP::Line;
     constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
@@ -93,13 +94,29 @@ __global__ void __launch_bounds__((P2Launch<T, N, false>::THREADS))
             for (int r = 0; r < RZ; ++r)
                 v[u * RZ + r] =
                     active ? work[base + P::template in_pos<RZ>(t, u, r)] : mk<T>(0, 0);
+        // the potential factors do not depend on the transform: issue their
+        // loads first so they land while the inverse FFT runs
+        constexpr bool PREFETCH = P::E <= 16;
+        c128 evp[PREFETCH ? P::E : 1];
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                for (int r = 0; r < RA; ++r)
+                    evp[u * RA + r] = active ? ev[base + P::template in_pos<RA>(t, u, r)]
+                                             : mk(0.0, 0.0);
+        }
         P::inverse(v, t, lds + l * P::LS, tw);
 #pragma unroll
         for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
             for (int r = 0; r < RA; ++r) {
                 const int pos = P::template in_pos<RA>(t, u, r);
-                if (active) v[u * RA + r] = apply(v[u * RA + r], ev[base + pos]);
+                if constexpr (PREFETCH) {
+                    v[u * RA + r] = apply(v[u * RA + r], evp[u * RA + r]);
+                } else {
+                    if (active) v[u * RA + r] = apply(v[u * RA + r], ev[base + pos]);
+                }
             }
     }
     if constexpr (MODE == SW_LAST) {
@@ -123,12 +140,12 @@ __global__ void __launch_bounds__((P2Launch<T, N, false>::THREADS))
 }
 
 // Column sweep over ncols columns of an nx(=N) x ncols array.
-template <typename T, int N>
-__global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
     k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
                 const cx<T> *__restrict__ tw)
 {
-    using LP = P2Launch<T, N, true>;
+    using LP = P2Launch<T, N, true, V>;
     using P = typename LP::Line;
     constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
@@ -136,7 +153,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
     const SplitDev &J = jobs[blockIdx.y];
     const int tid = threadIdx.x;
     const int l = tid % L, t = tid / L;
-    const size_t j = (size_t)blockIdx.x * L + l;
+    const size_t j = xcd_tile(blockIdx.x, gridDim.x) * L + l;
     const bool active = j < ncols;
     const size_t c = active ? j : 0;
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
@@ -262,6 +279,11 @@ struct qsim_split_job {
     SplitDev host{};
     SplitDev *dev = nullptr;  // device copy of `host`
     const void *tw_x = nullptr, *tw_y = nullptr;  // twiddles of the sweep kernels
+    // launch variants of the row / column kernels (autotuned or default 0)
+    int row_var = 0, col_var = 0;
+    // replay captured sweep sequences (keyed by step count)
+    bool use_graph = false;
+    std::map<size_t, hipGraphExec_t> graphs;
     // phase program
     int ev_start = 0, ev_end = 0, nph = 0;
     int ph_k[3] = {0, 0, 0}, ph_e[3] = {0, 0, 0};
@@ -336,12 +358,12 @@ int build_axis(qsim_split_job *j, c128 **out, size_t n, cd alpha, double a, doub
     return QSIM_OK;
 }
 
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, int V>
 int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const void *tw,
                hipStream_t s)
 {
-    using LP = P2Launch<T, N, false>;
-    auto fn = k_row_sweep<T, N, MODE>;
+    using LP = P2Launch<T, N, false, V>;
+    auto fn = k_row_sweep<T, N, MODE, V>;
     if (LP::LDS_BYTES > 65536)
         QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)LP::LDS_BYTES));
@@ -352,12 +374,12 @@ int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const voi
     return QSIM_OK;
 }
 
-template <typename T, int N>
+template <typename T, int N, int V>
 int launch_col(const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
                hipStream_t s)
 {
-    using LP = P2Launch<T, N, true>;
-    auto fn = k_col_sweep<T, N>;
+    using LP = P2Launch<T, N, true, V>;
+    auto fn = k_col_sweep<T, N, V>;
     if (LP::LDS_BYTES > 65536)
         QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)LP::LDS_BYTES));
@@ -402,21 +424,61 @@ int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps
     }                                                                                 \
     return set_error(QSIM_ERR_UNSUPPORTED, "no fused kernel for this length");
 
-int row_sweep(int prec, size_t ny, int mode, const SplitDev *jobs, int njobs, size_t nrows,
-              int iev, const void *tw, hipStream_t s)
+// Launch variants: lengths 1024 and 2048 have the candidates 0..kVariants-1
+// (P2Cfg), every other length only variant 0.
+constexpr int kVariants = 6;
+
+bool has_variants(size_t n) { return n == 1024 || n == 2048; }
+
+template <typename T, int NN, int MODE>
+int row_v(int var, const SplitDev *jobs, int njobs, size_t nrows, int iev, const void *tw,
+          hipStream_t s)
+{
+    if constexpr (NN == 1024 || NN == 2048) {
+        switch (var) {
+            case 1: return launch_row<T, NN, MODE, 1>(jobs, njobs, nrows, iev, tw, s);
+            case 2: return launch_row<T, NN, MODE, 2>(jobs, njobs, nrows, iev, tw, s);
+            case 3: return launch_row<T, NN, MODE, 3>(jobs, njobs, nrows, iev, tw, s);
+            case 4: return launch_row<T, NN, MODE, 4>(jobs, njobs, nrows, iev, tw, s);
+            case 5: return launch_row<T, NN, MODE, 5>(jobs, njobs, nrows, iev, tw, s);
+            default: break;
+        }
+    }
+    return launch_row<T, NN, MODE, 0>(jobs, njobs, nrows, iev, tw, s);
+}
+
+template <typename T, int NN>
+int col_v(int var, const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
+          hipStream_t s)
+{
+    if constexpr (NN == 1024 || NN == 2048) {
+        switch (var) {
+            case 1: return launch_col<T, NN, 1>(jobs, njobs, ncols, ik, tw, s);
+            case 2: return launch_col<T, NN, 2>(jobs, njobs, ncols, ik, tw, s);
+            case 3: return launch_col<T, NN, 3>(jobs, njobs, ncols, ik, tw, s);
+            case 4: return launch_col<T, NN, 4>(jobs, njobs, ncols, ik, tw, s);
+            case 5: return launch_col<T, NN, 5>(jobs, njobs, ncols, ik, tw, s);
+            default: break;
+        }
+    }
+    return launch_col<T, NN, 0>(jobs, njobs, ncols, ik, tw, s);
+}
+
+int row_sweep(int prec, size_t ny, int mode, int var, const SplitDev *jobs, int njobs,
+              size_t nrows, int iev, const void *tw, hipStream_t s)
 {
 #define ROWCALL(T, NN)                                                                     \
-    (mode == SW_FIRST ? launch_row<T, NN, SW_FIRST>(jobs, njobs, nrows, iev, tw, s)         \
-                      : mode == SW_MID ? launch_row<T, NN, SW_MID>(jobs, njobs, nrows, iev, tw, s) \
-                                       : launch_row<T, NN, SW_LAST>(jobs, njobs, nrows, iev, tw, s))
+    (mode == SW_FIRST ? row_v<T, NN, SW_FIRST>(var, jobs, njobs, nrows, iev, tw, s)         \
+                      : mode == SW_MID ? row_v<T, NN, SW_MID>(var, jobs, njobs, nrows, iev, tw, s) \
+                                       : row_v<T, NN, SW_LAST>(var, jobs, njobs, nrows, iev, tw, s))
     QSIM_SWITCH_N(prec, ny, ROWCALL)
 #undef ROWCALL
 }
 
-int col_sweep(int prec, size_t nx, const SplitDev *jobs, int njobs, size_t ncols, int ik,
-              const void *tw, hipStream_t s)
+int col_sweep(int prec, size_t nx, int var, const SplitDev *jobs, int njobs, size_t ncols,
+              int ik, const void *tw, hipStream_t s)
 {
-#define COLCALL(T, NN) launch_col<T, NN>(jobs, njobs, ncols, ik, tw, s)
+#define COLCALL(T, NN) col_v<T, NN>(var, jobs, njobs, ncols, ik, tw, s)
     QSIM_SWITCH_N(prec, nx, COLCALL)
 #undef COLCALL
 }
@@ -479,15 +541,16 @@ int fused_steps(qsim_split_job *j, const SplitDev *jobs, int njobs, size_t nstep
         return QSIM_OK;
     }
     const size_t nx = j->d.nx, ny = j->d.ny;
-    TIMED(tm, 0, row_sweep(prec, ny, SW_FIRST, jobs, njobs, nx, j->ev_start, j->tw_y, s));
+    const int rv = j->row_var, cv = j->col_var;
+    TIMED(tm, 0, row_sweep(prec, ny, SW_FIRST, rv, jobs, njobs, nx, j->ev_start, j->tw_y, s));
     for (size_t st = 0; st < nsteps; ++st) {
         for (int ph = 0; ph < j->nph; ++ph) {
-            TIMED(tm, 1, col_sweep(prec, nx, jobs, njobs, ny, j->ph_k[ph], j->tw_x, s));
+            TIMED(tm, 1, col_sweep(prec, nx, cv, jobs, njobs, ny, j->ph_k[ph], j->tw_x, s));
             const bool last = (st + 1 == nsteps) && (ph + 1 == j->nph);
             if (last)
-                TIMED(tm, 0, row_sweep(prec, ny, SW_LAST, jobs, njobs, nx, j->ev_end, j->tw_y, s));
+                TIMED(tm, 0, row_sweep(prec, ny, SW_LAST, rv, jobs, njobs, nx, j->ev_end, j->tw_y, s));
             else
-                TIMED(tm, 0, row_sweep(prec, ny, SW_MID, jobs, njobs, nx, j->ph_e[ph], j->tw_y, s));
+                TIMED(tm, 0, row_sweep(prec, ny, SW_MID, rv, jobs, njobs, nx, j->ph_e[ph], j->tw_y, s));
         }
     }
     return QSIM_OK;
@@ -558,9 +621,16 @@ int unfused_steps(qsim_split_job *j, size_t nsteps, hipStream_t s)
     return QSIM_OK;
 }
 
+void drop_graphs(qsim_split_job *j)
+{
+    for (auto &kv : j->graphs) (void)hipGraphExecDestroy(kv.second);
+    j->graphs.clear();
+}
+
 void destroy(qsim_split_job *j)
 {
     if (!j) return;
+    drop_graphs(j);
     for (void *p : j->owned) (void)hipFree(p);
     if (j->fwd) qsim_hip_fft_plan_destroy(j->fwd);
     if (j->inv) qsim_hip_fft_plan_destroy(j->inv);
@@ -744,8 +814,36 @@ int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
     QSIM_CHECK(j, "null job");
     if (nsteps == 0) return QSIM_OK;
     hipStream_t s = as_stream(stream);
-    if (j->fused) return fused_steps(j, j->dev, 1, nsteps, s);
-    return unfused_steps(j, nsteps, s);
+    if (!j->fused) return unfused_steps(j, nsteps, s);
+    if (!j->use_graph || j->d.dim != 2) return fused_steps(j, j->dev, 1, nsteps, s);
+    // the sweep sequence of `nsteps` steps is captured once into a HIP graph
+    // and replayed (one launch instead of 2*nsteps+1)
+    auto it = j->graphs.find(nsteps);
+    if (it == j->graphs.end()) {
+        hipStream_t cs;
+        QSIM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        int rc = QSIM_OK;
+        if (e == hipSuccess) {
+            rc = fused_steps(j, j->dev, 1, nsteps, cs);
+            hipError_t e2 = hipStreamEndCapture(cs, &g);
+            if (rc == QSIM_OK && e2 != hipSuccess) rc = hip_error(e2, "hipStreamEndCapture");
+        } else {
+            rc = hip_error(e, "hipStreamBeginCapture");
+        }
+        hipGraphExec_t ex = nullptr;
+        if (rc == QSIM_OK) {
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = hip_error(e, "hipGraphInstantiate");
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cs);
+        if (rc != QSIM_OK) return rc;
+        it = j->graphs.emplace(nsteps, ex).first;
+    }
+    QSIM_HIP(hipGraphLaunch(it->second, s));
+    return QSIM_OK;
 }
 
 int qsim_hip_split_steps_timed(qsim_split_job *j, size_t nsteps, double *kernel_ms,
@@ -769,6 +867,105 @@ int qsim_hip_split_steps_timed(qsim_split_job *j, size_t nsteps, double *kernel_
         kernel_ms[tm.kind[i]] += ms;
         kernel_launches[tm.kind[i]] += 1;
     }
+    return QSIM_OK;
+}
+
+int qsim_hip_split_use_graph(qsim_split_job *j, int enable)
+{
+    QSIM_CHECK(j, "null job");
+    j->use_graph = enable != 0;
+    if (!j->use_graph) drop_graphs(j);
+    return QSIM_OK;
+}
+
+int qsim_hip_split_set_variants(qsim_split_job *j, int row_variant, int col_variant)
+{
+    QSIM_CHECK(j, "null job");
+    QSIM_CHECK(row_variant >= 0 && row_variant < kVariants && col_variant >= 0 &&
+                   col_variant < kVariants,
+               "launch variant out of range");
+    j->row_var = has_variants(j->d.ny) ? row_variant : 0;
+    j->col_var = has_variants(j->d.nx) ? col_variant : 0;
+    drop_graphs(j);
+    return QSIM_OK;
+}
+
+int qsim_hip_split_autotune(qsim_split_job *j, int reps, int *row_variant, int *col_variant,
+                            double *row_us, double *col_us, qsim_stream_t stream)
+{
+    QSIM_CHECK(j, "null job");
+    QSIM_CHECK(reps >= 1, "reps must be >= 1");
+    if (!j->fused || j->d.dim != 2) {
+        if (row_variant) *row_variant = 0;
+        if (col_variant) *col_variant = 0;
+        return QSIM_OK;
+    }
+    hipStream_t s = as_stream(stream);
+    const int prec = j->d.precision;
+    const size_t esz = prec == 64 ? sizeof(c128) : sizeof(c64);
+    // time on a copy of the working state so the job's psi is untouched
+    void *scratch = nullptr, *tab = nullptr;
+    QSIM_HIP(hipMalloc(&scratch, j->n * esz));
+    hipError_t e = hipMemcpyAsync(scratch, j->work, j->n * esz, hipMemcpyDeviceToDevice, s);
+    SplitDev h = j->host;
+    h.work = scratch;
+    if (e == hipSuccess) e = hipMalloc(&tab, sizeof(SplitDev));
+    if (e == hipSuccess) e = hipMemcpy(tab, &h, sizeof(SplitDev), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(scratch);
+        if (tab) (void)hipFree(tab);
+        return hip_error(e, "autotune setup");
+    }
+    const SplitDev *jt = (const SplitDev *)tab;
+    hipEvent_t a, b;
+    QSIM_HIP(hipEventCreate(&a));
+    QSIM_HIP(hipEventCreate(&b));
+    int rc = QSIM_OK;
+    double best_r = 1e30, best_c = 1e30;
+    int vr = 0, vc = 0;
+    for (int which = 0; which < 2 && rc == QSIM_OK; ++which) {
+        const size_t nlen = which == 0 ? j->d.ny : j->d.nx;
+        const int nv = has_variants(nlen) ? kVariants : 1;
+        for (int v = 0; v < nv && rc == QSIM_OK; ++v) {
+            auto launch = [&]() {
+                return which == 0 ? row_sweep(prec, j->d.ny, SW_MID, v, jt, 1, j->d.nx,
+                                              j->ph_e[0], j->tw_y, s)
+                                  : col_sweep(prec, j->d.nx, v, jt, 1, j->d.ny, j->ph_k[0],
+                                              j->tw_x, s);
+            };
+            for (int w = 0; w < 2 && rc == QSIM_OK; ++w) rc = launch();
+            if (rc != QSIM_OK) break;
+            (void)hipEventRecord(a, s);
+            for (int r = 0; r < reps && rc == QSIM_OK; ++r) rc = launch();
+            (void)hipEventRecord(b, s);
+            if (rc != QSIM_OK) break;
+            (void)hipEventSynchronize(b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            const double us = 1e3 * ms / reps;
+            if (which == 0 && us < best_r) {
+                best_r = us;
+                vr = v;
+            }
+            if (which == 1 && us < best_c) {
+                best_c = us;
+                vc = v;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(scratch);
+    (void)hipFree(tab);
+    if (rc != QSIM_OK) return rc;
+    j->row_var = vr;
+    j->col_var = vc;
+    drop_graphs(j);
+    if (row_variant) *row_variant = vr;
+    if (col_variant) *col_variant = vc;
+    if (row_us) *row_us = best_r;
+    if (col_us) *col_us = best_c;
     return QSIM_OK;
 }
 
