@@ -15,7 +15,7 @@ namespace qsim {
 
 // ------------------------------------------------------------ twiddle tables
 // tw[m] = exp(-2 pi i m / n), m < n, from x87 long double, rounded once to
-// double and (for FP32) from the double to float.
+// double; FP32 tables hold the float high parts, then the float low parts.
 static std::mutex g_tw_mu;
 static std::map<std::pair<int, std::pair<size_t, int>>, void *> g_tw;
 
@@ -42,10 +42,14 @@ int twiddle_table(size_t n, int precision, const void **out)
         QSIM_HIP(hipMalloc(&d, n * sizeof(c128)));
         QSIM_HIP(hipMemcpy(d, h.data(), n * sizeof(c128), hipMemcpyHostToDevice));
     } else {
-        std::vector<float> f(2 * n);
-        for (size_t i = 0; i < 2 * n; ++i) f[i] = (float)h[i];
-        QSIM_HIP(hipMalloc(&d, n * sizeof(c64)));
-        QSIM_HIP(hipMemcpy(d, f.data(), n * sizeof(c64), hipMemcpyHostToDevice));
+        // n float high parts, then n float low parts (double - float)
+        std::vector<float> f(4 * n);
+        for (size_t i = 0; i < 2 * n; ++i) {
+            f[i] = (float)h[i];
+            f[2 * n + i] = (float)(h[i] - (double)f[i]);
+        }
+        QSIM_HIP(hipMalloc(&d, 2 * n * sizeof(c64)));
+        QSIM_HIP(hipMemcpy(d, f.data(), 2 * n * sizeof(c64), hipMemcpyHostToDevice));
     }
     g_tw[key] = d;
     *out = d;

@@ -31,7 +31,24 @@ template <>
 struct TwC<float> {
     static __device__ __forceinline__ float c(int k) { return kCos64f[k & 63]; }
     static __device__ __forceinline__ float s(int k) { return kSin64f[k & 63]; }
+    static __device__ __forceinline__ float cl(int k) { return kCos64fl[k & 63]; }
+    static __device__ __forceinline__ float sl(int k) { return kSin64fl[k & 63]; }
 };
+
+// FP32 data times a float-float factor (hi + lo): the factor carries ~48
+// bits, so a rounded constant never repeats its error coherently step after
+// step; only the products round (unbiased).
+__device__ __forceinline__ c64 mul_ff(c64 a, c64 h, c64 l)
+{
+    const float sr = a.x * l.x - a.y * l.y, si = a.x * l.y + a.y * l.x;
+    return mk(fmaf(a.x, h.x, fmaf(-a.y, h.y, sr)), fmaf(a.x, h.y, fmaf(a.y, h.x, si)));
+}
+// a * conj(h + l)
+__device__ __forceinline__ c64 mulconj_ff(c64 a, c64 h, c64 l)
+{
+    const float sr = a.x * l.x + a.y * l.y, si = a.y * l.x - a.x * l.y;
+    return mk(fmaf(a.x, h.x, fmaf(a.y, h.y, sr)), fmaf(a.y, h.x, fmaf(-a.x, h.y, si)));
+}
 
 // a * exp(sgn * 2 pi i * k / m), sgn = +1 for INV, -1 forward; k, m are
 // compile-time after unrolling, so every branch folds away.
@@ -45,13 +62,26 @@ __device__ __forceinline__ cx<T> twmul(cx<T> a, int k, int m)
     if (q == 48) return INV ? mk(a.y, -a.x) : mk(-a.y, a.x);
     const T c = TwC<T>::c(q);
     const T s = INV ? TwC<T>::s(q) : -TwC<T>::s(q);
-    if ((q & 15) == 8) {  // |c| = |s| = sqrt(1/2)
-        const T h = c < 0 ? -c : c;
-        const T pc = c < 0 ? -a.x : a.x, qc = c < 0 ? -a.y : a.y;   // c*a / h
-        const T ps = s < 0 ? -a.y : a.y, qs = s < 0 ? -a.x : a.x;   // s*a / h
-        return mk(h * (pc - ps), h * (qc + qs));
+    if constexpr (sizeof(T) == 4) {
+        const float cl = TwC<float>::cl(q);
+        const float sl = INV ? TwC<float>::sl(q) : -TwC<float>::sl(q);
+        if ((q & 15) == 8) {  // |c| = |s| = sqrt(1/2): (hi + lo) * (a.x -+ a.y)
+            const float h = c < 0 ? -c : c, hl = c < 0 ? -cl : cl;
+            const float pc = c < 0 ? -a.x : a.x, qc = c < 0 ? -a.y : a.y;
+            const float ps = s < 0 ? -a.y : a.y, qs = s < 0 ? -a.x : a.x;
+            const float d1 = pc - ps, d2 = qc + qs;
+            return mk(fmaf(h, d1, hl * d1), fmaf(h, d2, hl * d2));
+        }
+        return mul_ff(a, mk(c, s), mk(cl, sl));
+    } else {
+        if ((q & 15) == 8) {  // |c| = |s| = sqrt(1/2)
+            const T h = c < 0 ? -c : c;
+            const T pc = c < 0 ? -a.x : a.x, qc = c < 0 ? -a.y : a.y;   // c*a / h
+            const T ps = s < 0 ? -a.y : a.y, qs = s < 0 ? -a.x : a.x;   // s*a / h
+            return mk(h * (pc - ps), h * (qc + qs));
+        }
+        return mk(a.x * c - a.y * s, a.x * s + a.y * c);
     }
-    return mk(a.x * c - a.y * s, a.x * s + a.y * c);
 }
 
 __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
@@ -136,7 +166,7 @@ struct Pow2Line {
     // twiddle + butterfly for a pass of radix R after NS points combined.
     // FP64: one table load per butterfly, the powers w^r by multiplication
     // (error ~r ulp, far below the 1e-12 bar); FP32: every power from the
-    // correctly rounded table.
+    // table as a float-float pair.
     template <int R, int NS, bool INV>
     static __device__ __forceinline__ void pass(cx<T> *v, int t, const cx<T> *__restrict__ tw)
     {
@@ -155,10 +185,12 @@ struct Pow2Line {
                         if (r + 1 < R) w = w * w1;
                     }
                 } else {
+                    // FP32 tables hold n high parts, then n low parts
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
-                        cx<T> w = tw[step * r];
-                        v[u * R + r] = INV ? mulconj(v[u * R + r], w) : v[u * R + r] * w;
+                        const cx<T> wh = tw[step * r], wl = tw[N + step * r];
+                        v[u * R + r] = INV ? mulconj_ff(v[u * R + r], wh, wl)
+                                           : mul_ff(v[u * R + r], wh, wl);
                     }
                 }
             }
