@@ -111,13 +111,100 @@ def cpu_baseline(prob, steps):
 
 
 def traffic_from_profiles(name):
+    """HBM bytes per launch of `name` from the committed PMC summary, or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        return json.load(open(path)).get(name)
+        ent = json.load(open(path))["kernels"].get(name)
+        return None if ent is None else ent["bytes_per_launch"]
     except Exception:
         return None
+
+
+# kernels the PMC summary reports: (key, substring of the rocprof kernel name)
+PROBE_COPY_N = 1 << 20          # complex128 elements per calibration copy (16 MiB each way)
+PROBE_VARIANTS = (4, 4)         # row / column launch variants of the traffic probe
+
+
+def probe_kernels(rv, cv):
+    return (("copy", "qsim_hip_copy"),
+            ("row_sweep", "k_row_sweep<double, 1024, 1, %d>" % rv),
+            ("col_sweep", "k_col_sweep<double, 1024, %d>" % cv))
+
+
+def traffic_probe(steps):
+    """Workload of the rocprofv3 --pmc passes: 3 calibration copies of
+    PROBE_COPY_N complex128 (known bytes), then `steps` C2 steps with fixed
+    launch variants.  No timing, no autotuning."""
+    from qusim_amd import _native as Nn
+    from qusim_amd.workloads import c2_problem
+    Nn.init(0)
+    a = Nn.DeviceArray.from_host(np.zeros(PROBE_COPY_N, np.complex128))
+    b = Nn.DeviceArray.from_host(np.zeros(PROBE_COPY_N, np.complex128))
+    for _ in range(3):
+        Nn.call("qsim_hip_copy", b.ptr, a.ptr, PROBE_COPY_N, None)
+    Nn.synchronize()
+    job, keep = make_job(Nn, c2_problem(), 64)
+    Nn.call("qsim_hip_split_set_variants", job, PROBE_VARIANTS[0], PROBE_VARIANTS[1])
+    Nn.call("qsim_hip_split_steps", job, steps, None)
+    Nn.synchronize()
+    Nn.call("qsim_hip_split_destroy", job)
+    log("traffic probe: 3 copies of %d complex128, %d C2 steps, variants %s"
+        % (PROBE_COPY_N, steps, PROBE_VARIANTS))
+
+
+def _pmc_rows(d):
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under %s" % d)
+    rows = []
+    for f in files:
+        rows += list(csv.DictReader(open(f)))
+    return rows
+
+
+def _pmc_avg(rows, counter, sub):
+    """average over dispatches of kernels whose name contains `sub` (the
+    counter summed over its instances within one dispatch)."""
+    per = {}
+    for r in rows:
+        if r["Counter_Name"] != counter or sub not in r["Kernel_Name"]:
+            continue
+        key = r.get("Dispatch_Id") or r.get("Correlation_Id")
+        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    return (sum(per.values()) / len(per), len(per)) if per else (None, 0)
+
+
+def pmc_summary(fetch_dir, write_dir):
+    """Turn the FETCH_SIZE and WRITE_SIZE passes into profiles/traffic.json:
+    each counter (KiB per dispatch) scaled by what it reads for the copy
+    kernel of known size."""
+    fr, wr = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
+    copy_kib = PROBE_COPY_N * 16 / 1024.0
+    kern = probe_kernels(*PROBE_VARIANTS)
+    cf, nf = _pmc_avg(fr, "FETCH_SIZE", kern[0][1])
+    cw, nw = _pmc_avg(wr, "WRITE_SIZE", kern[0][1])
+    out = {"calibration": {"copy_bytes_each_way": copy_kib * 1024, "copy_fetch_kib": cf,
+                           "copy_write_kib": cw, "fetch_factor": copy_kib / cf,
+                           "write_factor": copy_kib / cw, "dispatches": [nf, nw]},
+           "variants": list(PROBE_VARIANTS), "kernels": {}}
+    alg = {"row_sweep": ROW_B_PER_PT[64] * (1 << 20), "col_sweep": COL_B_PER_PT[64] * (1 << 20)}
+    for key, sub in kern[1:]:
+        f, n1 = _pmc_avg(fr, "FETCH_SIZE", sub)
+        w, n2 = _pmc_avg(wr, "WRITE_SIZE", sub)
+        if f is None or w is None:
+            continue
+        fb = f * 1024 * out["calibration"]["fetch_factor"]
+        wb = w * 1024 * out["calibration"]["write_factor"]
+        out["kernels"][key] = {"fetch_bytes": fb, "write_bytes": wb, "bytes_per_launch": fb + wb,
+                               "algorithmic_bytes": alg[key], "ratio": (fb + wb) / alg[key],
+                               "raw_fetch_kib": f, "raw_write_kib": w, "dispatches": [n1, n2]}
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    json.dump(out, open(path, "w"), indent=1)
+    print(json.dumps(out, indent=1))
 
 
 def fp32_section(steps_err, steps_rate):
@@ -169,7 +256,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--fp32-err-steps", type=int, default=1000)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run the workload of the rocprofv3 --pmc passes and exit")
+    ap.add_argument("--probe-steps", type=int, default=6)
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc outputs (CPU)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
+    if args.traffic_probe:
+        traffic_probe(args.probe_steps)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(every box here has one GPU; see DESIGN.md for how the grid would shard)",

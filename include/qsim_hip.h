@@ -60,6 +60,10 @@ int qsim_hip_event_create(void **ev);
 int qsim_hip_event_destroy(void *ev);
 int qsim_hip_event_record(void *ev, qsim_stream_t stream);
 int qsim_hip_event_elapsed_ms(void *start, void *stop, float *ms);
+/* *done = 1 once the work before the event's record has completed, else 0
+ * (non-blocking); event_synchronize waits for it. */
+int qsim_hip_event_query(void *ev, int *done);
+int qsim_hip_event_synchronize(void *ev);
 
 /* ------------------------------------------------------- pointwise Device ops
  * Each replaces the DeviceTemplate<CudaExec> method of the same meaning
@@ -245,6 +249,18 @@ int qsim_hip_split_destroy(qsim_split_job *job);
 int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps,
                                  qsim_stream_t s);
 
+/* Reusable asynchronous batch (request loops): one launch per sweep for up
+ * to `capacity` same-shape fused jobs that share method, precision and
+ * launch variants.  batch_steps returns once the work is enqueued; the
+ * batch's descriptor table is reused by the next call, so one batch object
+ * serves one stream at a time.  Each job's result is bitwise the same as
+ * qsim_hip_split_steps on that job alone with the same nsteps. */
+typedef struct qsim_split_batch qsim_split_batch;
+int qsim_hip_split_batch_create(qsim_split_batch **batch, int capacity);
+int qsim_hip_split_batch_steps(qsim_split_batch *batch, qsim_split_job *const *jobs, int njobs,
+                               size_t nsteps, qsim_stream_t s);
+int qsim_hip_split_batch_destroy(qsim_split_batch *batch);
+
 /* ------------------------------------------------------ fused Born job
  * QuPerturbation2DImpl/3DImpl (src/Perturbation2DImpl.cpp,
  * src/Perturbation3DImpl.cpp) with the kernels of src/Perburbation.h. */
@@ -282,6 +298,29 @@ int qsim_hip_born_get_veff(qsim_born_job *job, int e, qsim_c128 *out, double *ep
 int qsim_hip_born_xsection(qsim_born_job *job, int e, double cosx, double cosy, double cosz,
                            double *dsigma, qsim_c128 *amp, qsim_stream_t s);
 int qsim_hip_born_destroy(qsim_born_job *job);
+
+/* ------------------------------------------------ input generation on the GPU
+ * Replace the reference's per-point host evaluation of the initial psi and
+ * the potential (std::function callbacks, src/EvolverImpl1D.cpp:40-70,
+ * src/EvolverImpl2D.cpp:40-99, src/Perturbation3DImpl.cpp:28-50) for the
+ * parameterised families of the benchmark workloads.  dim = 1..3; n, lo, hi
+ * (host, dim entries) give the grid x_a[i] = lo[a] + (hi[a]-lo[a])/n[a]*i.
+ *
+ * Potential: V = sum_b h_b exp(-|x - c_b|^2 / (2 w_b^2))
+ *               - i * absorb_strength * ((absorb_width - d)/absorb_width)^2
+ * for d < absorb_width (d = distance to the nearest box edge; absorb_width
+ * <= 0 disables the layer).  bumps (host): count x (h, w, c_0..c_{dim-1}),
+ * count <= 64.  Outputs: re and/or im (n doubles) and/or cv (complex128);
+ * NULL outputs are skipped. */
+int qsim_hip_gen_potential(int dim, const size_t *n, const double *lo, const double *hi,
+                           const double *bumps, int count, double absorb_width,
+                           double absorb_strength, double *re, double *im, qsim_c128 *cv,
+                           qsim_stream_t s);
+/* Packet psi = exp(i k.x - |x - center|^2 / (4 sigma^2)); normalise != 0
+ * scales it to sum |psi|^2 dV = 1 (compensated norm; synchronises s). */
+int qsim_hip_gen_packet(int dim, const size_t *n, const double *lo, const double *hi,
+                        const double *center, const double *k, double sigma, int normalise,
+                        qsim_c128 *psi, qsim_stream_t s);
 
 #ifdef __cplusplus
 }

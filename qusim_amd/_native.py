@@ -59,6 +59,8 @@ _SIGS = {
     "qsim_hip_event_destroy": [_VP],
     "qsim_hip_event_record": [_VP, _VP],
     "qsim_hip_event_elapsed_ms": [_VP, _VP, ctypes.POINTER(ctypes.c_float)],
+    "qsim_hip_event_query": [_VP, ctypes.POINTER(_I)],
+    "qsim_hip_event_synchronize": [_VP],
     "qsim_hip_set_zero": [_VP, _SZ, _VP],
     "qsim_hip_set_one": [_VP, _SZ, _VP],
     "qsim_hip_copy": [_VP, _VP, _SZ, _VP],
@@ -142,6 +144,11 @@ _SIGS = {
     "qsim_hip_born_xsection": [_VP, _I, _D, _D, _D, ctypes.POINTER(_D), ctypes.POINTER(c128),
                                _VP],
     "qsim_hip_born_destroy": [_VP],
+    "qsim_hip_split_batch_create": [_VP, _I],
+    "qsim_hip_split_batch_steps": [_VP, _VP, _I, _SZ, _VP],
+    "qsim_hip_split_batch_destroy": [_VP],
+    "qsim_hip_gen_potential": [_I, _VP, _VP, _VP, _VP, _I, _D, _D, _VP, _VP, _VP, _VP],
+    "qsim_hip_gen_packet": [_I, _VP, _VP, _VP, _VP, _VP, _D, _I, _VP, _VP],
 }
 
 _lib = None
@@ -261,6 +268,14 @@ class Event:
 
     def record(self, stream=None):
         call("qsim_hip_event_record", self.ptr, stream)
+
+    def query(self):
+        done = _I()
+        call("qsim_hip_event_query", self.ptr, ctypes.byref(done))
+        return bool(done.value)
+
+    def synchronize(self):
+        call("qsim_hip_event_synchronize", self.ptr)
 
     def elapsed_ms(self, later):
         ms = ctypes.c_float()

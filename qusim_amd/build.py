@@ -19,6 +19,7 @@ HIP_SOURCES = [
     "qsim_fft.hip",
     "qsim_split.hip",
     "qsim_born.hip",
+    "qsim_gen.hip",
 ]
 HOST_SOURCES = ["hip_device.cpp"]
 

@@ -649,7 +649,9 @@ int create(qsim_born_job *j)
         QSIM_TRY(balloc(j, &p, j->n * sizeof(c128)));
         j->last = (c128 *)p;
     }
-    QSIM_HIP(hipDeviceSynchronize());
+    // set-up ran on the null stream; work other streams already run is
+    // not waited for (request loops create jobs while others advance)
+    QSIM_HIP(hipStreamSynchronize(nullptr));
     return QSIM_OK;
 }
 

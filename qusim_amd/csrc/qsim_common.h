@@ -94,6 +94,18 @@ __host__ __device__ __forceinline__ cx<D> cvt(cx<S> a)
 }
 
 // fold_half(i, N) = i > N/2 ? N - i : i  (src/DeviceTemplate.h:544-547)
+// Non-temporal store for sweep outputs: the next sweep reads them from
+// other XCDs, so keeping the lines in this XCD's L2 gains nothing (measured:
+// alternating row / column passes run faster with them).
+template <typename T>
+__device__ __forceinline__ void store_nt(cx<T> *p, cx<T> v)
+{
+    // global address space: a global (not flat) store, no LDS counter wait
+    using GT = __attribute__((address_space(1))) T;
+    __builtin_nontemporal_store(v.x, (GT *)&p->x);
+    __builtin_nontemporal_store(v.y, (GT *)&p->y);
+}
+
 __host__ __device__ __forceinline__ size_t fold_half(size_t i, size_t n)
 {
     return (i > n / 2) ? (n - i) : i;

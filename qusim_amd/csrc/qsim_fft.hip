@@ -143,6 +143,76 @@ static void factorize(size_t n, GenFactors *gf)
     } while (m > 1 && gf->nf < 28);
 }
 
+// ------------------------------------------------------- four-step passes
+// Sub-lengths of the two passes: powers of two 64..4096.
+static bool four_sub_ok(size_t m) { return m >= 64 && m <= 4096 && !(m & (m - 1)); }
+
+template <typename T, int N, bool LF, bool INV, int PASS>
+static int launch_4s_inst(const void *in, void *out, size_t S, size_t nlines, size_t other,
+                          const void *tw, const void *twn, size_t nbig, double scale,
+                          hipStream_t s)
+{
+    using LP = P2Launch<T, N, LF>;
+    auto fn = k_fft_4step<T, N, LF, INV, PASS>;
+    if (LP::LDS_BYTES > 65536)
+        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)LP::LDS_BYTES));
+    size_t blocks = (nlines + LP::L - 1) / LP::L;
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(LP::THREADS), LP::LDS_BYTES, s,
+                       (const cx<T> *)in, (cx<T> *)out, S, nlines, other, (const cx<T> *)tw,
+                       (const cx<T> *)twn, nbig, (T)scale);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+// pass 0 reads with stride n1*S (line-fast tiles); pass 1 reads rows (S = 1)
+template <typename T, int N>
+static int launch_4s_n(int pass, bool inv, const void *in, void *out, size_t S, size_t nlines,
+                       size_t other, const void *tw, const void *twn, size_t nbig, double scale,
+                       hipStream_t s)
+{
+    if (pass == 0)
+        return inv ? launch_4s_inst<T, N, true, true, 0>(in, out, S, nlines, other, tw, twn,
+                                                         nbig, scale, s)
+                   : launch_4s_inst<T, N, true, false, 0>(in, out, S, nlines, other, tw, twn,
+                                                          nbig, scale, s);
+    return inv ? launch_4s_inst<T, N, false, true, 1>(in, out, S, nlines, other, tw, twn, nbig,
+                                                      scale, s)
+               : launch_4s_inst<T, N, false, false, 1>(in, out, S, nlines, other, tw, twn, nbig,
+                                                       scale, s);
+}
+
+static int launch_4s(int precision, size_t sub, int pass, bool inv, const void *in, void *out,
+                     size_t S, size_t nlines, size_t other, const void *tw, const void *twn,
+                     size_t nbig, double scale, hipStream_t s)
+{
+#define CASE_4S(NN)                                                                          \
+    case NN:                                                                                 \
+        return precision == 64 ? launch_4s_n<double, NN>(pass, inv, in, out, S, nlines, other, \
+                                                         tw, twn, nbig, scale, s)             \
+                               : launch_4s_n<float, NN>(pass, inv, in, out, S, nlines, other,  \
+                                                        tw, twn, nbig, scale, s);
+    switch (sub) {
+        CASE_4S(64)
+        CASE_4S(128)
+        CASE_4S(256)
+        CASE_4S(512)
+        CASE_4S(1024)
+        CASE_4S(2048)
+        CASE_4S(4096)
+        default:
+            break;
+    }
+#undef CASE_4S
+    return set_error(QSIM_ERR_UNSUPPORTED, "four-step sub-length not instantiated");
+}
+
+void release_axis(FftAxis *a)
+{
+    if (a->tmp) (void)hipFree(a->tmp);
+    a->tmp = nullptr;
+}
+
 int prepare_axis(FftAxis *a, size_t n, size_t S, size_t nlines, int precision)
 {
     a->n = n;
@@ -154,6 +224,21 @@ int prepare_axis(FftAxis *a, size_t n, size_t S, size_t nlines, int precision)
     size_t esz = precision == 64 ? sizeof(c128) : sizeof(c64);
     size_t per_line = 2 * n * esz;
     const size_t lds_max = 160 * 1024;
+    if (per_line > lds_max && !(n & (n - 1)) && S == 1) {
+        // a long power-of-two innermost axis: n1 = 2^floor(log2(n)/2), n2 = n / n1
+        size_t n1 = 1;
+        while (n1 * n1 * 4 <= n) n1 *= 2;
+        const size_t n2 = n / n1;
+        if (four_sub_ok(n1) && four_sub_ok(n2)) {
+            a->four = true;
+            a->n1 = n1;
+            a->n2 = n2;
+            QSIM_TRY(twiddle_table(n1, precision, &a->tw1));
+            QSIM_TRY(twiddle_table(n2, precision, &a->tw2));
+            QSIM_HIP(hipMalloc(&a->tmp, n * nlines * esz));
+            return QSIM_OK;
+        }
+    }
     if (per_line > lds_max)
         return set_error(QSIM_ERR_UNSUPPORTED,
                          "FFT length " + std::to_string(n) +
@@ -176,6 +261,14 @@ int run_axis(const FftAxis &a, int precision, bool inv, const void *in, void *ou
     // a length-1 axis is a copy/scale: the generic kernel with no factors
     if (a.n == 1 && in == out && scale == 1.0) return QSIM_OK;
     if (a.p2) return launch_fft_p2(precision, a.n, inv, in, out, a.S, a.nlines, a.tw, scale, s);
+    if (a.four) {
+        // pass 0: nlines*n1 lines of length n2 (stride n1), in -> tmp
+        QSIM_TRY(launch_4s(precision, a.n2, 0, inv, in, a.tmp, 1, a.nlines * a.n1, a.n1, a.tw2,
+                           a.tw, a.n, 1.0, s));
+        // pass 1: nlines*n2 rows of length n1, tmp -> out (transposed store)
+        return launch_4s(precision, a.n1, 1, inv, a.tmp, out, 1, a.nlines * a.n2, a.n2, a.tw1,
+                         a.tw, a.n, scale, s);
+    }
     size_t blocks = (a.nlines + a.L - 1) / a.L;
     static bool attr64 = false, attr32 = false;
     if (precision == 64) {
@@ -240,7 +333,7 @@ int qsim_hip_fft_plan_create(qsim_fft_plan **plan, int rank, const size_t *dims,
         for (int b = a + 1; b < rank; ++b) S *= dims[b];
         int rc = prepare_axis(&p->axis[a], dims[a], S, total / dims[a], precision);
         if (rc != QSIM_OK) {
-            delete p;
+            qsim_hip_fft_plan_destroy(p);
             return rc;
         }
     }
@@ -269,6 +362,8 @@ int qsim_hip_fft_exec(qsim_fft_plan *p, const void *in, void *out, qsim_stream_t
 
 int qsim_hip_fft_plan_destroy(qsim_fft_plan *p)
 {
+    if (p)
+        for (int a = 0; a < 3; ++a) release_axis(&p->axis[a]);
     delete p;
     return QSIM_OK;
 }

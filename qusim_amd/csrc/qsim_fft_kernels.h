@@ -37,6 +37,7 @@ QSIM_P2CFG(double, 2048, 3, 8, 16, 16, 1, 4)
 QSIM_P2CFG(double, 2048, 4, 16, 16, 8, 1, 8)
 QSIM_P2CFG(double, 2048, 5, 16, 16, 8, 4, 4)
 QSIM_P2CFG(double, 4096, 0, 16, 16, 16, 0, 0)
+QSIM_P2CFG(double, 8192, 0, 16, 16, 32, 0, 0)  // 1D split jobs only
 QSIM_P2CFG(float, 16, 0, 4, 4, 1, 0, 0)
 QSIM_P2CFG(float, 32, 0, 8, 4, 1, 0, 0)
 QSIM_P2CFG(float, 64, 0, 8, 8, 1, 0, 0)
@@ -57,6 +58,7 @@ QSIM_P2CFG(float, 2048, 4, 32, 32, 2, 1, 8)
 QSIM_P2CFG(float, 2048, 5, 16, 16, 8, 1, 4)
 QSIM_P2CFG(float, 4096, 0, 16, 16, 16, 0, 0)
 QSIM_P2CFG(float, 8192, 0, 16, 16, 32, 0, 0)
+QSIM_P2CFG(float, 16384, 0, 16, 32, 32, 0, 0)  // 1D split jobs only
 #undef QSIM_P2CFG
 
 __host__ __device__ constexpr int pow2floor(int x) { return x <= 1 ? 1 : 2 * pow2floor(x / 2); }
@@ -116,6 +118,83 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
 #pragma unroll
             for (int r = 0; r < R1; ++r)
                 out[base + (size_t)P::template in_pos<R1>(t, u, r) * S] = v[u * R1 + r] * scale;
+    }
+}
+
+// Four-step (Bailey) passes of one long axis of length n = n1 * n2 over a
+// tensor [outer][n][S]; element m = m1 + n1*m2, output k = k2 + n2*k1:
+//   X[k2 + n2 k1] = sum_m1 w_n1^(m1 k1) * [ w_n^(m1 k2) * sum_m2 x[m1 + n1 m2] w_n2^(m2 k2) ].
+// PASS 0: lines over (o, m1, c) of length N = n2, stride n1*S: FFT, times
+//         w_n^(+-m1 k2), stored in place of the inputs (in -> out).
+// PASS 1: lines over (o, k2, c) of length N = n1, stride S: FFT, stored at
+//         k2 + n2*k1 (stride n2*S), times `scale`.
+// `other` is n1 for PASS 0 and n2 for PASS 1; twn is the table of length n.
+template <typename T, int N, bool LF, bool INV, int PASS>
+__global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
+    k_fft_4step(const cx<T> *in, cx<T> *out, size_t S, size_t nlines, size_t other,
+                const cx<T> *__restrict__ tw, const cx<T> *__restrict__ twn, size_t nbig,
+                T scale)
+{
+    using LP = P2Launch<T, N, LF>;
+    using P = typename LP::Line;
+    constexpr int L = LP::L;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int l = LF ? tid % L : tid / P::TPL;
+    const int t = LF ? tid / L : tid % P::TPL;
+    const size_t g = (size_t)blockIdx.x * L + l;
+    const bool active = g < nlines;
+    const size_t gg = active ? g : 0;
+    // input stride: n1*S for pass 0, S for pass 1
+    const size_t SI = PASS == 0 ? other * S : S;
+    const size_t base = line_base(gg, N, SI);
+    cx<T> v[P::E];
+#pragma unroll
+    for (int u = 0; u < P::E / P::RA_; ++u)
+#pragma unroll
+        for (int r = 0; r < P::RA_; ++r)
+            v[u * P::RA_ + r] =
+                active ? in[base + (size_t)P::template in_pos<P::RA_>(t, u, r) * SI] : mk<T>(0, 0);
+    // every pass runs the forward radix order; the inverse is the conjugate
+    // DFT, taken by conjugating in and out (exact: a sign flip)
+    if constexpr (INV) {
+#pragma unroll
+        for (int e = 0; e < P::E; ++e) v[e].y = -v[e].y;
+    }
+    P::forward(v, t, lds + l * P::LS, tw);
+    constexpr int RZ = P::RLAST;
+    if (!active) return;
+    if constexpr (PASS == 0) {
+        const size_t m1 = (gg % (other * S)) / S;
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) {
+                const int k2 = P::template in_pos<RZ>(t, u, r);
+                const size_t m = m1 * (size_t)k2;  // < n1 * n2
+                cx<T> a = v[u * RZ + r];
+                if constexpr (sizeof(T) == 8) {
+                    a = a * twn[m];
+                } else {
+                    a = mul_ff(a, twn[m], twn[nbig + m]);
+                }
+                if constexpr (INV) a.y = -a.y;
+                out[base + (size_t)k2 * SI] = a;
+            }
+    } else {
+        const size_t row = gg / S, c = gg % S;       // row = o*n2 + k2
+        const size_t o = row / other, k2 = row % other;
+        const size_t obase = o * nbig * S + k2 * S + c;
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) {
+                const int k1 = P::template in_pos<RZ>(t, u, r);
+                cx<T> a = v[u * RZ + r];
+                if constexpr (INV) a.y = -a.y;
+                out[obase + (size_t)k1 * other * S] = a * scale;
+            }
     }
 }
 

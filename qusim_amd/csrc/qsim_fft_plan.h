@@ -15,11 +15,19 @@ struct FftAxis {
     size_t lds = 0;     // generic kernel: dynamic LDS bytes
     GenFactors gf{};
     const void *tw = nullptr;  // twiddle table exp(-2 pi i m / n), m < n
+    // four-step form of a long innermost axis, n = n1 * n2 (two passes
+    // through a scratch tensor of the whole size)
+    bool four = false;
+    size_t n1 = 0, n2 = 0;
+    const void *tw1 = nullptr, *tw2 = nullptr;
+    void *tmp = nullptr;  // owned by the plan that prepared the axis
 };
 
 int twiddle_table(size_t n, int precision, const void **out);
 bool is_p2_supported(size_t n, int precision);
 int prepare_axis(FftAxis *a, size_t n, size_t S, size_t nlines, int precision);
+// free what prepare_axis allocated (the four-step scratch)
+void release_axis(FftAxis *a);
 // Transform every line of the axis, in -> out, multiplying by `scale`.
 int run_axis(const FftAxis &a, int precision, bool inv, const void *in, void *out, double scale,
              hipStream_t s);

@@ -214,4 +214,24 @@ int qsim_hip_event_elapsed_ms(void *start, void *stop, float *ms)
     return QSIM_OK;
 }
 
+int qsim_hip_event_query(void *ev, int *done)
+{
+    QSIM_CHECK(ev && done, "null event");
+    const hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(ev));
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return QSIM_OK;
+    }
+    QSIM_HIP(e);
+    *done = 1;
+    return QSIM_OK;
+}
+
+int qsim_hip_event_synchronize(void *ev)
+{
+    QSIM_CHECK(ev, "null event");
+    QSIM_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(ev)));
+    return QSIM_OK;
+}
+
 }  // extern "C"

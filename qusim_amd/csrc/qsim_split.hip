@@ -125,7 +125,8 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
             for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
                 for (int r = 0; r < RA; ++r)
-                    J.psi[base + P::template in_pos<RA>(t, u, r)] = conv<double>(v[u * RA + r]);
+                    store_nt(&J.psi[base + P::template in_pos<RA>(t, u, r)],
+                             conv<double>(v[u * RA + r]));
         }
     } else {
         P::forward(v, t, lds + l * P::LS, tw);
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
             for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
                 for (int r = 0; r < RZ; ++r)
-                    work[base + P::template in_pos<RZ>(t, u, r)] = v[u * RZ + r];
+                    store_nt(&work[base + P::template in_pos<RZ>(t, u, r)], v[u * RZ + r]);
         }
     }
 }
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
         for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
             for (int r = 0; r < RA; ++r)
-                work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols] = v[u * RA + r];
+                store_nt(&work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols], v[u * RA + r]);
     }
 }
 
@@ -486,15 +487,22 @@ int col_sweep(int prec, size_t nx, int var, const SplitDev *jobs, int njobs, siz
 int run_1d(int prec, size_t n, const SplitDev *jobs, int njobs, const Prog1D &prog,
            size_t nsteps, const void *tw, hipStream_t s)
 {
+    // one workgroup holds the whole line: 8192 (FP64) and 16384 (FP32) fill
+    // 139 KB of LDS, so they exist for the persistent 1D kernel only
+    if (prec == 64 && n == 8192) return launch_1d<double, 8192>(jobs, njobs, prog, nsteps, tw, s);
+    if (prec == 32 && n == 16384) return launch_1d<float, 16384>(jobs, njobs, prog, nsteps, tw, s);
 #define ONECALL(T, NN) launch_1d<T, NN>(jobs, njobs, prog, nsteps, tw, s)
     QSIM_SWITCH_N(prec, n, ONECALL)
 #undef ONECALL
 }
 
-bool fused_len_ok(size_t n, int prec)
+// lengths with a fused kernel: 2D axes 64..4096 (FP64) / 8192 (FP32); 1D
+// lines up to twice that
+bool fused_len_ok(size_t n, int prec, int dim)
 {
     if (n < 64 || (n & (n - 1))) return false;
-    return prec == 64 ? n <= 4096 : n <= 8192;
+    const size_t cap = (prec == 64 ? 4096 : 8192) * (dim == 1 ? 2 : 1);
+    return n <= cap;
 }
 
 // Optional per-kernel timing: HIP events around every sweep launch.
@@ -646,12 +654,12 @@ int create(qsim_split_job *j, const double *Vre, const double *Vim)
     const double dx = (d.x1 - d.x0) / d.nx;
     const double dy = d.dim == 2 ? (d.y1 - d.y0) / d.ny : 1.0;
     const cd dt(d.dt_re, d.dt_im);
-    j->fused = d.boundary == 0 && fused_len_ok(d.nx, d.precision) &&
-               (d.dim == 1 || fused_len_ok(d.ny, d.precision));
+    j->fused = d.boundary == 0 && fused_len_ok(d.nx, d.precision, d.dim) &&
+               (d.dim == 1 || fused_len_ok(d.ny, d.precision, d.dim));
     if (!j->fused && d.precision != 64)
         return set_error(QSIM_ERR_UNSUPPORTED,
-                         "FP32-mixed split job needs power-of-two axes (64..8192) and a "
-                         "periodic boundary");
+                         "FP32-mixed split job needs power-of-two axes (2D: 64..8192, 1D: "
+                         "64..16384) and a periodic boundary");
     // potential factors
     SplitDev &h = j->host;
     h.psi = j->psi;
@@ -773,7 +781,9 @@ int create(qsim_split_job *j, const double *Vre, const double *Vim)
     QSIM_TRY(dalloc(j, &p, sizeof(SplitDev)));
     j->dev = (SplitDev *)p;
     QSIM_HIP(hipMemcpy(j->dev, &h, sizeof(SplitDev), hipMemcpyHostToDevice));
-    QSIM_HIP(hipDeviceSynchronize());
+    // set-up ran on the null stream; work other streams already run is
+    // not waited for (request loops create jobs while others advance)
+    QSIM_HIP(hipStreamSynchronize(nullptr));
     return QSIM_OK;
 }
 
@@ -1024,6 +1034,73 @@ int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps
     QSIM_TRY(fused_steps(j0, table, njobs, nsteps, s));
     // the host vector must outlive the async copy
     QSIM_HIP(hipStreamSynchronize(s));
+    return QSIM_OK;
+}
+
+// ------------------------------------------------ reusable asynchronous batch
+struct qsim_split_batch {
+    int cap = 0;
+    SplitDev *table = nullptr;   // device
+    SplitDev *staged = nullptr;  // pinned host
+    hipEvent_t copied = nullptr; // the last table upload has been read
+    bool pending = false;
+};
+
+int qsim_hip_split_batch_create(qsim_split_batch **b, int capacity)
+{
+    QSIM_CHECK(b && capacity > 0, "batch: null out-pointer or capacity < 1");
+    *b = nullptr;
+    qsim_split_batch *p = new qsim_split_batch();
+    p->cap = capacity;
+    int rc = QSIM_OK;
+    if (hipMalloc(&p->table, sizeof(SplitDev) * capacity) != hipSuccess ||
+        hipHostMalloc(&p->staged, sizeof(SplitDev) * capacity, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&p->copied, hipEventDisableTiming) != hipSuccess)
+        rc = set_error(QSIM_ERR_NOMEM, "batch: allocation failed");
+    if (rc != QSIM_OK) {
+        qsim_hip_split_batch_destroy(p);
+        return rc;
+    }
+    *b = p;
+    return QSIM_OK;
+}
+
+int qsim_hip_split_batch_steps(qsim_split_batch *b, qsim_split_job *const *jobs, int njobs,
+                               size_t nsteps, qsim_stream_t stream)
+{
+    QSIM_CHECK(b && jobs && njobs >= 0, "batch: null argument");
+    QSIM_CHECK(njobs <= b->cap, "batch: more jobs than the batch's capacity");
+    if (njobs == 0 || nsteps == 0) return QSIM_OK;
+    qsim_split_job *j0 = jobs[0];
+    QSIM_CHECK(j0 && j0->fused, "batch: jobs must be fused split jobs");
+    for (int i = 1; i < njobs; ++i) {
+        const qsim_split_job *ji = jobs[i];
+        QSIM_CHECK(ji && ji->fused && ji->d.dim == j0->d.dim && ji->d.nx == j0->d.nx &&
+                       ji->d.ny == j0->d.ny && ji->d.method == j0->d.method &&
+                       ji->d.precision == j0->d.precision && ji->row_var == j0->row_var &&
+                       ji->col_var == j0->col_var,
+                   "batch: jobs differ in shape, method, precision or launch variants");
+    }
+    hipStream_t s = as_stream(stream);
+    // the pinned staging area may still feed the previous upload
+    if (b->pending) QSIM_HIP(hipEventSynchronize(b->copied));
+    for (int i = 0; i < njobs; ++i) b->staged[i] = jobs[i]->host;
+    QSIM_HIP(hipMemcpyAsync(b->table, b->staged, sizeof(SplitDev) * njobs,
+                            hipMemcpyHostToDevice, s));
+    QSIM_HIP(hipEventRecord(b->copied, s));
+    b->pending = true;
+    return fused_steps(j0, b->table, njobs, nsteps, s);
+}
+
+int qsim_hip_split_batch_destroy(qsim_split_batch *b)
+{
+    if (!b) return QSIM_OK;
+    if (b->pending) (void)hipEventSynchronize(b->copied);
+    if (b->copied) (void)hipEventDestroy(b->copied);
+    if (b->staged) (void)hipHostFree(b->staged);
+    if (b->table) (void)hipFree(b->table);
+    delete b;
     return QSIM_OK;
 }
 

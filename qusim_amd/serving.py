@@ -1,0 +1,566 @@
+"""Request loop over one MI355X: continuous batching of split-operator
+evolutions (1D and 2D) and 2D Born-series scattering solves.
+
+The reference runs one solver per caller, synchronously, with a private
+Device each (src/EvolverImpl.h:25); this loop keeps many solvers resident on
+the device at once instead:
+
+* Requests arrive over time.  A request is admitted (its grids built on the
+  GPU and its job created) as soon as its HBM footprint fits the budget,
+  first come first served.
+* Admitted split jobs join a lane of their shape: (kind, n, precision).
+  A lane advances all its jobs in one launch per sweep
+  (qsim_hip_split_batch_steps) on its own stream, chunk by chunk.  New jobs
+  join the lane between chunks, and finished jobs leave it, while the other
+  lanes keep running.
+* Born solves run on a worker thread with a stream of their own; each solve
+  advances a fixed number of iterations per turn.
+* A job advances in chunks of a fixed length set by its kind and size
+  (`quantum`), never by what else is on the device.  The batched kernels
+  compute every job independently.  So a result is bitwise the same alone
+  and in any mix.
+
+Footprints (bytes per grid point), which bound what a job allocates:
+  evolution FP32-mixed: psi 16 + potential staging 16 + phase tables 32 + FP32 state 8 = 72
+  evolution FP64 fused: 64; FP64 unfused (lengths without a fused kernel): 192
+  Born FP32: V 16 + iterate 8 + work 8 = 32; Born FP64: 48 (+ small per-axis tables)
+"""
+import collections
+import ctypes
+import math
+import queue
+import threading
+import time
+
+import numpy as np
+
+from . import _native as N
+from .workloads import MT19937_64, SEED
+
+KINDS = ("evo1d", "evo2d", "born2d")
+
+
+# ------------------------------------------------------------------ requests
+class Request:
+    """One solve: kind, points per axis n, work (steps, or the iteration cap
+    of a Born solve), arrival time (s) and the physical parameters."""
+    __slots__ = ("rid", "kind", "n", "work", "arrival", "params")
+
+    def __init__(self, rid, kind, n, work, arrival, params):
+        self.rid, self.kind, self.n, self.work = rid, kind, int(n), int(work)
+        self.arrival, self.params = float(arrival), params
+
+    def points(self):
+        return self.n if self.kind == "evo1d" else self.n * self.n
+
+    def __repr__(self):
+        return "Request(%d, %s, n=%d, work=%d, t=%.4f)" % (self.rid, self.kind, self.n,
+                                                            self.work, self.arrival)
+
+
+def make_requests(count=2000, rate=400.0, seed=SEED + 5, mix=(0.6, 0.3, 0.1)):
+    """Seeded stream with Poisson arrivals at `rate` requests/s.
+
+    * 1D evolutions (share mix[0]): N in {1024..16384}, steps U{100..2000}.
+      Barrier h exp(-x^2/2) with h ~ U(0.3,0.9) on [-100,100), packet at -30
+      with sigma 5 and k ~ U(0.8,1.2), dt 0.5 (the 1D tunnelling set-up).
+    * 2D evolutions (mix[1]): n in {256,512,1024}, steps U{50..500}.
+      16 Gaussian bumps plus the absorbing layer on [-150,150)^2, packet
+      sigma 8 at (-60,0), k (0.8,0), dt 0.2 (the 2D benchmark set-up, with
+      the request's own bumps).
+    * 2D Born scattering (mix[2]): n in {256,512,1024} on [-80,80)^2.  8 bumps
+      a ~ U(0,0.3), w ~ U(1.5,4), centres in [-10,10]^2; incident along x
+      with k ~ U(1,3), E = k^2/2.  Vellekoop preconditioner, slow 0.5; up to
+      200 iterations, stopping once the relative update is below 1e-6."""
+    g = MT19937_64(seed)
+    t = 0.0
+    out = []
+    c1, c2 = mix[0], mix[0] + mix[1]
+    for i in range(count):
+        t += -math.log(1.0 - g.uniform()) / rate
+        u = g.uniform()
+        if u < c1:
+            n = 1024 << int(g.uniform() * 5)
+            steps = 100 + int(g.uniform() * 1901)
+            params = dict(h=g.uniform(0.3, 0.9), k=g.uniform(0.8, 1.2), center=-30.0, sigma=5.0,
+                          dt=0.5, lo=-100.0, hi=100.0)
+            out.append(Request(i, "evo1d", n, steps, t, params))
+        elif u < c2:
+            n = (256, 512, 1024)[int(g.uniform() * 3)]
+            steps = 50 + int(g.uniform() * 451)
+            bumps = []
+            for _ in range(16):  # drawn in the order (cx, cy, h, w)
+                cx, cy = g.uniform(-40, 40), g.uniform(-40, 40)
+                h, w = g.uniform(0, 1), g.uniform(2, 6)
+                bumps.append((h, w, cx, cy))
+            params = dict(bumps=bumps, dt=0.2, lo=-150.0, hi=150.0, center=(-60.0, 0.0),
+                          k=(0.8, 0.0), sigma=8.0, absorb=(20.0, 0.1))
+            out.append(Request(i, "evo2d", n, steps, t, params))
+        else:
+            n = (256, 512, 1024)[int(g.uniform() * 3)]
+            bumps = []
+            for _ in range(8):
+                a, w = g.uniform(0, 0.3), g.uniform(1.5, 4)
+                bumps.append((a, w, g.uniform(-10, 10), g.uniform(-10, 10)))
+            k = g.uniform(1, 3)
+            params = dict(bumps=bumps, energy=0.5 * k * k, lo=-80.0, hi=80.0, tol=1e-6,
+                          slow=0.5)
+            out.append(Request(i, "born2d", n, 200, t, params))
+    return out
+
+
+def footprint(req, precision):
+    """HBM bytes a request's job holds at most (see the module notes)."""
+    pts = req.points()
+    if req.kind == "born2d":
+        return pts * (32 if precision == 32 else 48) + (1 << 20)
+    if precision == 32:
+        return pts * 72 + (1 << 20)
+    fused = req.n <= (8192 if req.kind == "evo1d" else 4096)
+    return pts * (64 if fused else 192) + (1 << 20)
+
+
+def default_quantum(req):
+    """Steps (or iterations) per turn: about a millisecond of device time,
+    fixed by the request's own kind and size."""
+    if req.kind == "evo1d":
+        return 200
+    if req.kind == "evo2d":
+        return {256: 100, 512: 50}.get(req.n, 25)
+    return 8
+
+
+# --------------------------------------------------------------- device jobs
+def _arr(v, t=ctypes.c_double):
+    return (t * len(v))(*v)
+
+
+class _Staging:
+    """Reusable potential buffers per size: a split job reads its potential
+    only while it is created, so one pair per size serves every request."""
+
+    def __init__(self):
+        self.bufs = {}
+
+    def get(self, pts):
+        if pts not in self.bufs:
+            self.bufs[pts] = (N.DeviceArray(pts, np.float64), N.DeviceArray(pts, np.float64))
+        return self.bufs[pts]
+
+    def close(self):
+        for a, b in self.bufs.values():
+            a.close()
+            b.close()
+        self.bufs = {}
+
+
+class EvoJob:
+    """A split-operator evolution resident on the device."""
+
+    def __init__(self, req, precision, staging):
+        self.req, self.precision = req, precision
+        p = req.params
+        pts = req.points()
+        self.psi = N.DeviceArray(pts, np.complex128)
+        vre, vim = staging.get(pts)
+        d = N.SplitDesc()
+        if req.kind == "evo1d":
+            dims, lo, hi = [req.n], [p["lo"]], [p["hi"]]
+            N.call("qsim_hip_gen_packet", 1, _arr(dims, ctypes.c_size_t), _arr(lo), _arr(hi),
+                   _arr([p["center"]]), _arr([p["k"]]), p["sigma"], 1, self.psi.ptr, None)
+            N.call("qsim_hip_gen_potential", 1, _arr(dims, ctypes.c_size_t), _arr(lo), _arr(hi),
+                   _arr([p["h"], 1.0, 0.0]), 1, 0.0, 0.0, vre.ptr, vim.ptr, None, None)
+            d.dim, d.nx, d.ny = 1, req.n, 1
+            d.x0, d.x1, d.y0, d.y1 = p["lo"], p["hi"], 0.0, 1.0
+        else:
+            dims, lo, hi = [req.n, req.n], [p["lo"]] * 2, [p["hi"]] * 2
+            flat = [x for b in p["bumps"] for x in b]
+            N.call("qsim_hip_gen_packet", 2, _arr(dims, ctypes.c_size_t), _arr(lo), _arr(hi),
+                   _arr(p["center"]), _arr(p["k"]), p["sigma"], 1, self.psi.ptr, None)
+            N.call("qsim_hip_gen_potential", 2, _arr(dims, ctypes.c_size_t), _arr(lo), _arr(hi),
+                   _arr(flat), len(p["bumps"]), p["absorb"][0], p["absorb"][1], vre.ptr,
+                   vim.ptr, None, None)
+            d.dim, d.nx, d.ny = 2, req.n, req.n
+            d.x0, d.x1, d.y0, d.y1 = p["lo"], p["hi"], p["lo"], p["hi"]
+        d.dt_re, d.dt_im = p["dt"], 0.0
+        d.mass = d.hbar = 1.0
+        d.method, d.boundary, d.precision = 2, 0, precision
+        h = ctypes.c_void_p()
+        N.call("qsim_hip_split_create", ctypes.byref(h), ctypes.byref(d), vre.ptr, vim.ptr,
+               self.psi.ptr)
+        self.job = h.value
+        self.done = 0
+        self.dv = ((p["hi"] - p["lo"]) / req.n) ** (1 if req.kind == "evo1d" else 2)
+
+    def next_chunk(self, q):
+        return min(q, self.req.work - self.done)
+
+    def finished(self):
+        return self.done >= self.req.work
+
+    def result(self, keep):
+        """norm after the evolution (and psi when kept); reads psi on the
+        null stream, which the lanes' non-blocking streams do not wait on."""
+        out = {"norm": N.reduce1("qsim_hip_norm2", self.psi.ptr, self.psi.size) * self.dv}
+        if keep:
+            out["psi"] = self.psi.download()
+        return out
+
+    def close(self):
+        if self.job:
+            N.call("qsim_hip_split_destroy", self.job)
+            self.job = None
+        self.psi.close()
+
+
+# Born cross sections are reported toward these directions (angle from the
+# incident x axis): forward, 30 degrees, 90 degrees.
+BORN_ANGLES = (0.0, math.pi / 6, math.pi / 2)
+
+
+class BornJob:
+    """A 2D Born-series solve resident on the device."""
+
+    def __init__(self, req, precision):
+        self.req, self.precision = req, precision
+        p = req.params
+        pts = req.points()
+        self.V = N.DeviceArray(pts, np.complex128)
+        dims, lo, hi = [req.n, req.n], [p["lo"]] * 2, [p["hi"]] * 2
+        flat = [x for b in p["bumps"] for x in b]
+        N.call("qsim_hip_gen_potential", 2, _arr(dims, ctypes.c_size_t), _arr(lo), _arr(hi),
+               _arr(flat), len(p["bumps"]), 0.0, 0.0, None, None, self.V.ptr, None)
+        d = N.BornDesc()
+        d.dim = 2
+        d.n[0], d.n[1], d.n[2] = req.n, req.n, 1
+        d.lo[0], d.lo[1], d.lo[2] = p["lo"], p["lo"], 0.0
+        d.hi[0], d.hi[1], d.hi[2] = p["hi"], p["hi"], 1.0
+        d.direction[0], d.direction[1], d.direction[2] = 1.0, 0.0, 0.0
+        d.mass = d.hbar = 1.0
+        d.preconditional, d.preconditioner, d.slow = 1, 1, p["slow"]
+        d.epsilon, d.precision = 0.0, precision
+        h = ctypes.c_void_p()
+        E = ctypes.c_double(p["energy"])
+        N.call("qsim_hip_born_create", ctypes.byref(h), ctypes.byref(d), self.V.ptr,
+               ctypes.byref(E), 1)
+        self.job = h.value
+        self.done = 0
+        self.resid = float("inf")
+        self.tol = p["tol"]
+
+    def next_chunk(self, q):
+        return min(q, self.req.work - self.done)
+
+    def finished(self):
+        return self.done >= self.req.work or self.resid < self.tol
+
+    def advance(self, q, stream):
+        it, r = ctypes.c_int(), ctypes.c_double()
+        N.call("qsim_hip_born_iterate", self.job, self.next_chunk(q), self.tol, ctypes.byref(it),
+               ctypes.byref(r), stream)
+        self.done += it.value
+        self.resid = r.value
+
+    def result(self, keep, stream=None):
+        out = {"iterations": self.done, "residual": self.resid, "dsigma": []}
+        for th in BORN_ANGLES:
+            ds, amp = ctypes.c_double(), N.c128()
+            N.call("qsim_hip_born_xsection", self.job, 0, math.cos(th), math.sin(th), 0.0,
+                   ctypes.byref(ds), ctypes.byref(amp), stream)
+            out["dsigma"].append(ds.value)
+        if keep:
+            buf = N.DeviceArray(self.req.points(), np.complex128)
+            N.call("qsim_hip_born_get_dpsi", self.job, 0, buf.ptr, stream)
+            N.synchronize(stream)
+            out["dpsi"] = buf.download()
+            buf.close()
+        return out
+
+    def close(self):
+        if self.job:
+            N.call("qsim_hip_born_destroy", self.job)
+            self.job = None
+        self.V.close()
+
+
+class _Lane:
+    """Same-shape split jobs advanced together on one stream."""
+
+    CAPACITY = 4096
+
+    def __init__(self, quantum):
+        self.quantum = quantum
+        p = ctypes.c_void_p()
+        N.call("qsim_hip_stream_create", ctypes.byref(p))
+        self.stream = p.value
+        b = ctypes.c_void_p()
+        N.call("qsim_hip_split_batch_create", ctypes.byref(b), self.CAPACITY)
+        self.batch = b.value
+        self.event = N.Event()
+        self.jobs = []
+        self.inflight = None
+
+    def launch(self):
+        groups = collections.OrderedDict()
+        for j in self.jobs:
+            groups.setdefault(j.next_chunk(self.quantum), []).append(j)
+        self.inflight = []
+        for chunk, js in groups.items():
+            for k in range(0, len(js), self.CAPACITY):
+                part = js[k:k + self.CAPACITY]
+                arr = (ctypes.c_void_p * len(part))(*[j.job for j in part])
+                N.call("qsim_hip_split_batch_steps", self.batch, arr, len(part), chunk,
+                       self.stream)
+            self.inflight.append((js, chunk))
+        self.event.record(self.stream)
+
+    def poll(self):
+        """jobs whose last chunk completed and are finished (removed)."""
+        if self.inflight is None or not self.event.query():
+            return None
+        done = []
+        for js, chunk in self.inflight:
+            for j in js:
+                j.done += chunk
+                if j.finished():
+                    done.append(j)
+        self.inflight = None
+        if done:
+            ids = set(id(j) for j in done)
+            self.jobs = [j for j in self.jobs if id(j) not in ids]
+        return done
+
+    def close(self):
+        if self.inflight is not None:
+            self.event.synchronize()
+        N.call("qsim_hip_split_batch_destroy", self.batch)
+        N.call("qsim_hip_stream_destroy", self.stream)
+
+
+# -------------------------------------------------------------------- server
+class Server:
+    """Continuous-batching request loop on device 0.
+
+    precision: 32 (FP32-mixed) or 64.  hbm_budget: bytes of job footprints
+    admitted at once.  realtime: requests become visible at their arrival
+    times (otherwise all at t = 0).  keep: request ids whose full fields
+    (psi / dpsi) are returned."""
+
+    def __init__(self, precision=32, hbm_budget=32 << 30, realtime=True, keep=(),
+                 quantum=default_quantum):
+        self.precision = precision
+        self.budget = int(hbm_budget)
+        self.realtime = realtime
+        self.keep = set(keep)
+        self.quantum = quantum
+        self._lock = threading.Lock()
+        self._used = 0
+        self.peak_used = 0
+        self.records = {}
+        self.results = {}
+        self.errors = {}
+        self._ndone = 0
+
+    # footprint accounting (shared with the Born worker)
+    def _reserve(self, fp):
+        with self._lock:
+            if self._used + fp > self.budget:
+                return False
+            self._used += fp
+            self.peak_used = max(self.peak_used, self._used)
+            return True
+
+    def _release(self, fp):
+        with self._lock:
+            self._used -= fp
+
+    def _finish(self, rid, t0):
+        with self._lock:
+            self.records[rid]["done"] = time.perf_counter() - t0
+            self._ndone += 1
+
+    def _born_worker(self, inbox, t0):
+        p = ctypes.c_void_p()
+        N.call("qsim_hip_stream_create", ctypes.byref(p))
+        stream = p.value
+        active = []
+        try:
+            while True:
+                try:
+                    item = inbox.get(timeout=0.002) if not active else inbox.get_nowait()
+                except queue.Empty:
+                    item = None
+                if item is not None:
+                    if item == "stop":
+                        if not active:
+                            break
+                        inbox.put("stop")
+                    else:
+                        req = item
+                        try:
+                            active.append(BornJob(req, self.precision))
+                        except Exception as e:  # reported, never retried
+                            self.errors[req.rid] = str(e)
+                            self._finish(req.rid, t0)
+                            self._release(footprint(req, self.precision))
+                    continue
+                for job in list(active):
+                    job.advance(self.quantum(job.req), stream)
+                    if job.finished():
+                        self.results[job.req.rid] = job.result(job.req.rid in self.keep, stream)
+                        self._finish(job.req.rid, t0)
+                        job.close()
+                        self._release(footprint(job.req, self.precision))
+                        active.remove(job)
+        finally:
+            for job in active:
+                job.close()
+            N.call("qsim_hip_stream_destroy", stream)
+
+    def run(self, requests):
+        """Serve `requests`; returns the statistics (see `summary`)."""
+        N.init(0)
+        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        pending = collections.deque(reqs)
+        waiting = collections.deque()
+        lanes = {}
+        graveyard = []
+        staging = _Staging()
+        inbox = queue.Queue()
+        t0 = time.perf_counter()
+        worker = threading.Thread(target=self._born_worker, args=(inbox, t0), daemon=True)
+        worker.start()
+        total = len(reqs)
+
+        def flush():
+            for j in graveyard:
+                j.close()
+                self._release(footprint(j.req, self.precision))
+            graveyard.clear()
+
+        try:
+            while self._ndone < total:
+                now = time.perf_counter() - t0
+                while pending and (not self.realtime or pending[0].arrival <= now):
+                    waiting.append(pending.popleft())
+                # admission, first come first served, by footprint
+                while waiting:
+                    r = waiting[0]
+                    fp = footprint(r, self.precision)
+                    if fp > self.budget:
+                        waiting.popleft()
+                        self.errors[r.rid] = "footprint %d B exceeds the HBM budget" % fp
+                        self.records[r.rid] = {"arrival": r.arrival, "admitted": now}
+                        self._finish(r.rid, t0)
+                        continue
+                    if not self._reserve(fp):
+                        if graveyard:
+                            flush()
+                            continue
+                        break
+                    waiting.popleft()
+                    self.records[r.rid] = {"arrival": r.arrival if self.realtime else 0.0,
+                                           "admitted": time.perf_counter() - t0}
+                    if r.kind == "born2d":
+                        inbox.put(r)
+                        continue
+                    try:
+                        job = EvoJob(r, self.precision, staging)
+                    except Exception as e:
+                        self.errors[r.rid] = str(e)
+                        self._finish(r.rid, t0)
+                        self._release(fp)
+                        continue
+                    key = (r.kind, r.n)
+                    if key not in lanes:
+                        lanes[key] = _Lane(self.quantum(r))
+                    lanes[key].jobs.append(job)
+                busy = False
+                for lane in lanes.values():
+                    done = lane.poll()
+                    if done:
+                        for j in done:
+                            self.results[j.req.rid] = j.result(j.req.rid in self.keep)
+                            self._finish(j.req.rid, t0)
+                            graveyard.append(j)
+                    if lane.inflight is None and lane.jobs:
+                        lane.launch()
+                    busy = busy or lane.inflight is not None
+                if not busy and graveyard:
+                    flush()
+                if not busy:
+                    if pending and not waiting and inbox.empty():
+                        wait = pending[0].arrival - (time.perf_counter() - t0)
+                        time.sleep(min(max(wait, 0.0), 0.01))
+                    else:
+                        time.sleep(0.0002)
+                else:
+                    time.sleep(0.00005)
+        finally:
+            inbox.put("stop")
+            worker.join()
+            for lane in lanes.values():
+                lane.close()
+            flush()
+            staging.close()
+        self.makespan = time.perf_counter() - t0
+        return self.summary(reqs)
+
+    def summary(self, reqs):
+        lat = {k: [] for k in KINDS}
+        qd = []
+        for r in reqs:
+            rec = self.records[r.rid]
+            if r.rid in self.errors:
+                continue
+            lat[r.kind].append(rec["done"] - rec["arrival"])
+            qd.append(rec["admitted"] - rec["arrival"])
+        allv = sorted(v for k in KINDS for v in lat[k])
+
+        def pct(v, p):
+            return float(np.percentile(v, p)) if v else None
+
+        first = min(self.records[r.rid]["arrival"] for r in reqs)
+        last = max(self.records[r.rid]["done"] for r in reqs)
+        return {
+            "requests": len(reqs), "completed": len(allv), "errors": len(self.errors),
+            "solves_per_s": len(allv) / (last - first) if last > first else None,
+            "makespan_s": self.makespan,
+            "latency_p50_s": pct(allv, 50), "latency_p99_s": pct(allv, 99),
+            "queue_delay_p50_s": pct(qd, 50), "queue_delay_p99_s": pct(qd, 99),
+            "per_kind": {k: {"count": len(lat[k]), "p50_s": pct(lat[k], 50),
+                             "p99_s": pct(lat[k], 99)} for k in KINDS},
+            "peak_footprint_bytes": self.peak_used, "hbm_budget_bytes": self.budget,
+            "precision": self.precision,
+        }
+
+
+def solve_alone(req, precision=32, keep=True):
+    """One request served by itself (arrival at t = 0): its result record."""
+    r = Request(req.rid, req.kind, req.n, req.work, 0.0, req.params)
+    s = Server(precision=precision, realtime=False, keep=(req.rid,) if keep else ())
+    s.run([r])
+    if req.rid in s.errors:
+        raise RuntimeError(s.errors[req.rid])
+    return s.results[req.rid]
+
+
+def compare(kind, got, ref):
+    """error of an FP32-mixed result against its FP64 twin: psi relative L2
+    (evolutions) or the largest relative error of the Born cross sections."""
+    if kind == "born2d":
+        return max(abs(a - b) / abs(b) for a, b in zip(got["dsigma"], ref["dsigma"]))
+    a, b = got["psi"], ref["psi"]
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def spot_check(requests, results, every=50):
+    """Every `every`-th request re-solved alone in FP64; per-kind errors."""
+    errs = {k: [] for k in KINDS}
+    for r in requests[::every]:
+        if r.rid not in results:
+            continue
+        ref = solve_alone(r, precision=64)
+        errs[r.kind].append(compare(r.kind, results[r.rid], ref))
+    return {k: {"checked": len(v), "max": max(v) if v else None,
+                "median": float(np.median(v)) if v else None} for k, v in errs.items()}

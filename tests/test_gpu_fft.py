@@ -56,3 +56,38 @@ def test_fft_in_place():
     d = DeviceArray.from_host(x)
     FourierTransform(x.shape).transform(d, d)
     assert rel_l2(d.download(), np.fft.fftn(x)) < 1e-13
+
+
+@pytest.mark.parametrize("shape,prec", [((8192,), 64), ((16384,), 64), ((65536,), 64),
+                                        ((3, 16384), 64), ((16384,), 32), ((32768,), 32)],
+                         ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else str(v))
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fft_long_axis_four_step(shape, prec, inverse, oracle):
+    """Power-of-two innermost axes beyond one workgroup's LDS run as two
+    passes (n = n1 * n2 with a twiddle between and a transposed store)."""
+    from qusim_amd import fftn
+    x = _rand(shape, 11)
+    if prec == 32:
+        x = x.astype(np.complex64)
+        got = fftn(x, inverse=inverse, precision=32)
+        ref = oracle.fft(x.astype(np.complex128), inverse=inverse)
+        assert rel_l2(got, ref) < 2e-6
+    else:
+        got = fftn(x, inverse=inverse, precision=64)
+        ref = oracle.fft(x, inverse=inverse)
+        assert rel_l2(got, ref) < 1e-13
+
+
+def test_fft_long_axis_delta_and_round_trip():
+    """A shifted delta transforms to one exponential exactly: pins the
+    four-step twiddle index and the transposed store position by position."""
+    from qusim_amd import fftn
+    n, m0 = 16384, 37
+    x = np.zeros(n, np.complex128)
+    x[m0] = 1.0
+    got = fftn(x)
+    k = np.arange(n)
+    ref = np.exp(-2j * np.pi * ((k * m0) % n) / n)
+    assert np.max(np.abs(got - ref)) < 1e-13
+    y = fftn(got, inverse=True)
+    assert np.max(np.abs(y - n * x)) < 1e-9

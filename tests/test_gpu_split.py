@@ -64,8 +64,11 @@ def test_c1_tunnelling_matches_oracle(oracle):
     assert abs(e.Norm2(0.0, 100.0) - oracle.norm2_range_1d(ref, -100, 100, 0, 100)) < 1e-12
 
 
-@pytest.mark.parametrize("n,method", [(256, 4), (4096, 2), (64, 2)])
+@pytest.mark.parametrize("n,method", [(256, 4), (4096, 2), (64, 2), (8192, 2), (16384, 2),
+                                      (16384, 4)])
 def test_1d_fused_matches_oracle(n, method, oracle):
+    """fused persistent 1D kernel up to 8192 (FP64); 16384 runs the unfused
+    order on the four-step FFT"""
     x = -50 + 100.0 / n * np.arange(n)
     psi = np.exp(1.3j * x - (x + 10) ** 2 / 20)
     V = 0.4 * np.exp(-x ** 2 / 3)
@@ -163,3 +166,18 @@ def test_norm_conserved_for_real_potential():
         from qusim_amd._native import call
         call("qsim_hip_split_steps", e._job, 100, None)
     assert abs(e.Norm2() / n0 - 1) < 1e-12
+
+
+@pytest.mark.parametrize("n", [8192, 16384])
+def test_1d_fp32_long_lines(n, oracle):
+    """FP32-mixed persistent 1D kernel at 8192 and 16384 points (one
+    workgroup holds the line): error vs the FP64 oracle after 500 steps."""
+    x = -100 + 200.0 / n * np.arange(n)
+    psi = np.exp(1.0j * x - (x + 30) ** 2 / 100)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * (200.0 / n))
+    V = 0.6 * np.exp(-x ** 2 / 2)
+    e = run_1d(psi, V, -100.0, 100.0, 0.1, 500, 2, precision="mixed32", batch=100)
+    ref = oracle.split1d(psi, V, -100.0, 100.0, 0.1, 500, method=2)
+    err = rel_l2(e.GetPsi(), ref)
+    print("1D FP32 n=%d 500 steps rel L2 %.2e" % (n, err))
+    assert err < 1e-5
