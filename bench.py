@@ -61,8 +61,8 @@ def autotune(job, reps=20):
     ru, cu = ctypes.c_double(), ctypes.c_double()
     Nn.call("qsim_hip_split_autotune", job, reps, ctypes.byref(vr), ctypes.byref(vc),
             ctypes.byref(ru), ctypes.byref(cu), None)
-    return {"row_variant": vr.value, "col_variant": vc.value, "row_us": ru.value,
-            "col_us": cu.value}
+    return {"row_variant": vr.value, "col_variant": vc.value,
+            "pair_us": cu.value}  # one alternating row + column sweep pair
 
 
 def time_steps(job, steps, warmup):

@@ -233,10 +233,13 @@ int qsim_hip_split_steps_timed(qsim_split_job *job, size_t nsteps, double *kerne
  * than direct launches); enable = 1 turns it on. */
 int qsim_hip_split_use_graph(qsim_split_job *job, int enable);
 /* Launch variants of the 2D sweep kernels (radix set, lines per workgroup):
- * 0..5 for axis lengths 1024 and 2048, only 0 otherwise. */
+ * 0..7 for axis lengths 1024 and 2048 (6 and 7 run four radix passes with
+ * 8 elements per thread), only 0 otherwise. */
 int qsim_hip_split_set_variants(qsim_split_job *job, int row_variant, int col_variant);
-/* Autotuner: times every row / column variant (reps launches each, on a copy
- * of the state), keeps the fastest on the job and reports it.  Synchronous. */
+/* Autotuner: times every row variant, then every column variant, each in reps
+ * alternating (row, column) sweep pairs on a copy of the state, keeps the
+ * fastest on the job and reports it.  *row_us / *col_us: microseconds per
+ * pair at the end of the row and of the column stage.  Synchronous. */
 int qsim_hip_split_autotune(qsim_split_job *job, int reps, int *row_variant, int *col_variant,
                             double *row_us, double *col_us, qsim_stream_t s);
 /* Re-read psi after the caller changed it (e.g. renormalised it). */

@@ -83,18 +83,57 @@ __device__ __forceinline__ c128 source(c128 dpsi, c128 p0, c128 v, double eps)
     else return mk(v.x, v.y + eps) * dpsi + v.x * p0;                // op_Add3
 }
 
+// Launch variant of the Born sweeps per line length: 128-point lines run
+// 8 elements per thread (radices 8, 8, 2), which leaves registers for the
+// prefetched operands of the inner sweep; other lengths use variant 0.
+template <int N>
+struct BornV {
+    static constexpr int V = N == 128 ? 1 : 0;
+};
+
+// Sum over the workgroup: wave reduction by lane shuffles, then one value
+// per wave through LDS.  Every thread must call it.
+template <int THREADS>
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *slots)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    constexpr int NW = (THREADS + 63) / 64;
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __syncthreads();
+    if (lane == 0) {
+        slots[w] = a;
+        slots[NW + w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sb = 0;
+        for (int k = 0; k < NW; ++k) {
+            sa += slots[k];
+            sb += slots[NW + k];
+        }
+        a = sa;
+        b = sb;
+    }
+}
+
 // Inner sweep: lines along the last axis.  Line g -> outer indices
-// (i, j) = (g / nmid, g % nmid) in 3D, i = g in 2D.
+// (i, j) = (g / nmid, g % nmid) in 3D, i = g in 2D.  The operands of the
+// update (dpsi, V) are loaded before the inverse transform, so their latency
+// hides behind it.
 template <typename T, int N, int MODE, int PRE>
-__global__ void __launch_bounds__((P2Launch<T, N, false>::THREADS))
+__global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
     k_born_inner(const BornDev *__restrict__ jobs, const c128 *__restrict__ V, BornGeom geo,
                  const cx<T> *__restrict__ tw)
 {
-    using LP = P2Launch<T, N, false>;
+    using LP = P2Launch<T, N, false, BornV<N>::V>;
     using P = typename LP::Line;
     constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    __shared__ double slots[2 * ((LP::THREADS + 63) / 64)];
     const BornDev &J = jobs[blockIdx.y];
     const int tid = threadIdx.x;
     const int l = tid / P::TPL, t = tid % P::TPL;
@@ -108,50 +147,54 @@ __global__ void __launch_bounds__((P2Launch<T, N, false>::THREADS))
     cx<T> *dp = reinterpret_cast<cx<T> *>(J.dpsi);
     constexpr int RA = P::RA_, RZ = P::RLAST;
     const int dim = geo.dim;
-    // element (axis indices) of the inner position k
-    auto ijk = [&](int k, size_t &a, size_t &b, size_t &c) {
-        if (dim == 3) {
-            a = i;
-            b = j;
-            c = (size_t)k;
-        } else {
-            a = i;
-            b = (size_t)k;
-            c = 0;
-        }
-    };
     cx<T> v[P::E];
-    double rsum = 0, nsum = 0;
     if constexpr (MODE != BA_START) {
 #pragma unroll
         for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
                 v[u * RZ + r] = active ? tmp[base + P::template in_pos<RZ>(t, u, r)] : mk<T>(0, 0);
-        P::inverse(v, t, lds + l * P::LS, tw);
     }
+    c128 vv[P::E];
+    cx<T> dd[P::E];
 #pragma unroll
     for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
         for (int r = 0; r < RA; ++r) {
             const int k = P::template in_pos<RA>(t, u, r);
-            size_t a, b, c;
-            ijk(k, a, b, c);
-            const c128 vv = active ? V[base + k] : mk(0.0, 0.0);
-            const c128 ve = PRE == PRE_NAIVE ? vv : v_eff(vv, J, dim, a, b, c);
-            c128 d = active ? cvt<double>(dp[base + k]) : mk(0.0, 0.0);
+            vv[u * RA + r] = active ? V[base + k] : mk(0.0, 0.0);
+            dd[u * RA + r] = active ? dp[base + k] : mk<T>(0, 0);
+        }
+    if constexpr (MODE != BA_START) P::inverse(v, t, lds + l * P::LS, tw);
+    // per-line factors of the separable tables; the inner index varies below
+    const double ab_line = dim == 3 ? J.ab[0][i] + J.ab[1][j] : J.ab[0][i];
+    const c128 pw_line = dim == 3 ? J.pw[0][i] * J.pw[1][j] : J.pw[0][i];
+    const double *ab_in = J.ab[dim == 3 ? 2 : 1];
+    const c128 *pw_in = J.pw[dim == 3 ? 2 : 1];
+    double rsum = 0, nsum = 0;
+#pragma unroll
+    for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+        for (int r = 0; r < RA; ++r) {
+            const int k = P::template in_pos<RA>(t, u, r);
+            const c128 vk = vv[u * RA + r];
+            // v_eff: (ab0 + ab1) + ab2 in 3D, ab0 + ab1 in 2D (reference order)
+            const c128 ve = PRE == PRE_NAIVE ? vk : mk(vk.x, vk.y + (ab_line + ab_in[k]));
+            c128 d = cvt<double>(dd[u * RA + r]);
             if constexpr (MODE != BA_START) {
                 const c128 tn = cvt<double>(v[u * RA + r]) * J.inv_n;
                 const c128 nd = update<PRE>(d, tn, ve, J.slow, J.eps);
                 const cx<T> nds = cvt<T>(nd);
-                if (active) dp[base + k] = nds;
+                if (active) store_nt(&dp[base + k], nds);
                 const c128 ndr = cvt<double>(nds);
-                rsum += abs2(ndr - d);
-                nsum += abs2(ndr);
+                if (active) {
+                    rsum += abs2(ndr - d);
+                    nsum += abs2(ndr);
+                }
                 d = ndr;
             }
             if constexpr (MODE != BA_END) {
-                const c128 p0 = psi0_at(J, dim, a, b, c);
+                const c128 p0 = pw_line * pw_in[k];
                 v[u * RA + r] = cvt<T>(source<PRE>(d, p0, ve, J.eps));
             }
         }
@@ -162,26 +205,15 @@ __global__ void __launch_bounds__((P2Launch<T, N, false>::THREADS))
             for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
                 for (int r = 0; r < RZ; ++r)
-                    tmp[base + P::template in_pos<RZ>(t, u, r)] = v[u * RZ + r];
+                    store_nt(&tmp[base + P::template in_pos<RZ>(t, u, r)], v[u * RZ + r]);
         }
     }
     if constexpr (MODE != BA_START) {
         // block sums of the update residual and the new norm
-        __shared__ double red[2][1024];
-        __syncthreads();
-        red[0][tid] = rsum;
-        red[1][tid] = nsum;
-        __syncthreads();
-        for (int w = LP::THREADS / 2; w > 0; w >>= 1) {
-            if (tid < w) {
-                red[0][tid] += red[0][tid + w];
-                red[1][tid] += red[1][tid + w];
-            }
-            __syncthreads();
-        }
+        block_sum2<LP::THREADS>(rsum, nsum, slots);
         if (tid == 0) {
-            J.resid[2 * blockIdx.x] = red[0][0];
-            J.resid[2 * blockIdx.x + 1] = red[1][0];
+            J.resid[2 * blockIdx.x] = rsum;
+            J.resid[2 * blockIdx.x + 1] = nsum;
         }
     }
 }
@@ -235,7 +267,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
         for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
             for (int r = 0; r < RA; ++r)
-                tmp[c + (size_t)P::template in_pos<RA>(t, u, r) * S] = v[u * RA + r];
+                store_nt(&tmp[c + (size_t)P::template in_pos<RA>(t, u, r) * S], v[u * RA + r]);
     }
 }
 
@@ -276,7 +308,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
         for (int u = 0; u < P::E / R1; ++u)
 #pragma unroll
             for (int r = 0; r < R1; ++r)
-                tmp[base + (size_t)P::template in_pos<R1>(t, u, r) * S] = v[u * R1 + r];
+                store_nt(&tmp[base + (size_t)P::template in_pos<R1>(t, u, r) * S], v[u * R1 + r]);
     }
 }
 
@@ -381,7 +413,7 @@ template <typename T, int N, int MODE, int PRE>
 int launch_inner_t(const BornDev *tab, int nact, const c128 *V, const BornGeom &geo,
                    const void *tw, hipStream_t s)
 {
-    using LP = P2Launch<T, N, false>;
+    using LP = P2Launch<T, N, false, BornV<N>::V>;
     auto fn = k_born_inner<T, N, MODE, PRE>;
     if (LP::LDS_BYTES > 65536)
         QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -397,7 +429,7 @@ int launch_inner_t(const BornDev *tab, int nact, const c128 *V, const BornGeom &
 template <typename T, int N>
 size_t inner_blocks_t(const BornGeom &geo)
 {
-    using LP = P2Launch<T, N, false>;
+    using LP = P2Launch<T, N, false, BornV<N>::V>;
     return (geo.n0 * geo.nmid + LP::L - 1) / LP::L;
 }
 

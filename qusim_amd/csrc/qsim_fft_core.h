@@ -3,10 +3,10 @@
 // * dft_reg<T, R, INV>: an R-point DFT (R = 2..64, power of two) held in one
 //   thread's registers: radix-2 decimation in frequency with compile-time
 //   twiddles, output permuted back to natural order by register renaming.
-// * Pow2Line<T, N, RA, RB, RC>: one length-N line transformed by N/RMAX
-//   threads in up to three Stockham passes (radices RA, RB, RC), data in
-//   registers, one LDS exchange between passes.  Forward runs RA, RB, RC;
-//   inverse runs RC, RB, RA, so the forward's outputs are exactly the
+// * Pow2Line<T, N, RA, RB, RC, RD>: one length-N line transformed by N/RMAX
+//   threads in up to four Stockham passes (radices RA, RB, RC, RD), data in
+//   registers, one LDS exchange between passes.  Forward runs RA..RD;
+//   inverse runs RD..RA, so the forward's outputs are exactly the
 //   inverse's inputs in every thread's registers (and vice versa): an
 //   FFT -> pointwise -> IFFT sequence needs no exchange around the pointwise
 //   step.
@@ -136,17 +136,20 @@ __device__ __forceinline__ void dft_reg(cx<T> *v)
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
-// One length-N line, N = RA*RB*RC (RB, RC may be 1).  Thread t < TPL holds
-// E = RMAX elements; for a pass of radix R it owns butterflies
-// j = t + u*TPL, u < E/R, element r of butterfly u in v[u*R + r].
-template <typename T, int N, int RA, int RB, int RC>
+// One length-N line, N = RA*RB*RC*RD (RB, RC, RD may be 1; RD > 1 needs
+// RC > 1).  Thread t < TPL holds E = RMAX elements; for a pass of radix R it
+// owns butterflies j = t + u*TPL, u < E/R, element r of butterfly u in
+// v[u*R + r].  More passes mean smaller E: more threads (waves) per line.
+template <typename T, int N, int RA, int RB, int RC, int RD = 1>
 struct Pow2Line {
-    static_assert(RA * RB * RC == N, "radices must multiply to N");
-    static constexpr int NP = RC > 1 ? 3 : (RB > 1 ? 2 : 1);
+    static_assert(RA * RB * RC * RD == N, "radices must multiply to N");
+    static_assert(RD == 1 || RC > 1, "a fourth pass needs a third");
+    static constexpr int NP = RD > 1 ? 4 : (RC > 1 ? 3 : (RB > 1 ? 2 : 1));
     static constexpr int RA_ = RA;
-    static constexpr int E = cmax(RA, cmax(RB, RC));
+    static constexpr int E = cmax(cmax(RA, RB), cmax(RC, RD));
     static constexpr int TPL = N / E;
-    static constexpr int RMIN = cmin(RA, cmin(RB > 1 ? RB : RA, RC > 1 ? RC : RA));
+    static constexpr int RMIN = cmin(cmin(RA, RB > 1 ? RB : RA),
+                                     cmin(RC > 1 ? RC : RA, RD > 1 ? RD : RA));
     // LDS elements per line: N plus one pad per RMIN, rounded to 1 mod 16 so
     // lines placed side by side start on different banks.
     static constexpr int LS_RAW = N + N / RMIN + 1;
@@ -219,7 +222,7 @@ struct Pow2Line {
     }
 
     // Forward transform.  In: inputs of radix-RA pass (in_pos<RA>).  Out: X at
-    // in_pos<R_last>(t, u, r) with R_last = RC, RB or RA (NP = 3, 2, 1).
+    // in_pos<RLAST>(t, u, r) with RLAST = RD, RC, RB or RA (NP = 4, 3, 2, 1).
     static __device__ __forceinline__ void forward(cx<T> *v, int t, cx<T> *lds,
                                                    const cx<T> *__restrict__ tw)
     {
@@ -228,13 +231,17 @@ struct Pow2Line {
             exchange<RA, 1, RB>(v, t, lds);
             pass<RB, RA, false>(v, t, tw);
         }
-        if constexpr (NP == 3) {
+        if constexpr (NP >= 3) {
             exchange<RB, RA, RC>(v, t, lds);
             pass<RC, RA * RB, false>(v, t, tw);
         }
+        if constexpr (NP == 4) {
+            exchange<RC, RA * RB, RD>(v, t, lds);
+            pass<RD, RA * RB * RC, false>(v, t, tw);
+        }
     }
 
-    static constexpr int RLAST = NP == 3 ? RC : (NP == 2 ? RB : RA);
+    static constexpr int RLAST = NP == 4 ? RD : (NP == 3 ? RC : (NP == 2 ? RB : RA));
 
     // Inverse transform, radices RLAST, ..., RA.  In: in_pos<RLAST>.  Out: x at
     // in_pos<RA>(t, u, r).
@@ -247,12 +254,20 @@ struct Pow2Line {
             pass<RB, 1, true>(v, t, tw);
             exchange<RB, 1, RA>(v, t, lds);
             pass<RA, RB, true>(v, t, tw);
-        } else {
+        } else if constexpr (NP == 3) {
             pass<RC, 1, true>(v, t, tw);
             exchange<RC, 1, RB>(v, t, lds);
             pass<RB, RC, true>(v, t, tw);
             exchange<RB, RC, RA>(v, t, lds);
             pass<RA, RC * RB, true>(v, t, tw);
+        } else {
+            pass<RD, 1, true>(v, t, tw);
+            exchange<RD, 1, RC>(v, t, lds);
+            pass<RC, RD, true>(v, t, tw);
+            exchange<RC, RD, RB>(v, t, lds);
+            pass<RB, RD * RC, true>(v, t, tw);
+            exchange<RB, RD * RC, RA>(v, t, lds);
+            pass<RA, RD * RC * RB, true>(v, t, tw);
         }
     }
 };

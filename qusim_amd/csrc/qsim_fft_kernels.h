@@ -12,16 +12,18 @@ namespace qsim {
 // (contiguous) and column (strided) kernels; 0 = derive from LDS.
 template <typename T, int N, int V = 0>
 struct P2Cfg;
-#define QSIM_P2CFG(TYPE, N_, V_, A, B, C_, RL, CL)      \
-    template <>                                         \
-    struct P2Cfg<TYPE, N_, V_> {                        \
-        using Line = Pow2Line<TYPE, N_, A, B, C_>;      \
-        static constexpr int ROW_L = RL, COL_L = CL;    \
+#define QSIM_P2CFG4(TYPE, N_, V_, A, B, C_, D_, RL, CL) \
+    template <>                                          \
+    struct P2Cfg<TYPE, N_, V_> {                         \
+        using Line = Pow2Line<TYPE, N_, A, B, C_, D_>;   \
+        static constexpr int ROW_L = RL, COL_L = CL;     \
     };
+#define QSIM_P2CFG(TYPE, N_, V_, A, B, C_, RL, CL) QSIM_P2CFG4(TYPE, N_, V_, A, B, C_, 1, RL, CL)
 QSIM_P2CFG(double, 16, 0, 4, 4, 1, 0, 0)
 QSIM_P2CFG(double, 32, 0, 8, 4, 1, 0, 0)
 QSIM_P2CFG(double, 64, 0, 8, 8, 1, 0, 0)
 QSIM_P2CFG(double, 128, 0, 16, 8, 1, 0, 0)
+QSIM_P2CFG(double, 128, 1, 8, 8, 2, 0, 0)  // 8 per thread (Born sweeps)
 QSIM_P2CFG(double, 256, 0, 16, 16, 1, 0, 0)
 QSIM_P2CFG(double, 512, 0, 8, 8, 8, 0, 0)
 QSIM_P2CFG(double, 1024, 0, 16, 16, 4, 1, 4)
@@ -30,18 +32,24 @@ QSIM_P2CFG(double, 1024, 2, 32, 32, 1, 2, 4)
 QSIM_P2CFG(double, 1024, 3, 32, 32, 1, 2, 8)
 QSIM_P2CFG(double, 1024, 4, 8, 8, 16, 1, 4)
 QSIM_P2CFG(double, 1024, 5, 16, 16, 4, 2, 2)
+// four passes, 8 elements per thread: two waves per line
+QSIM_P2CFG4(double, 1024, 6, 8, 8, 4, 4, 1, 4)
+QSIM_P2CFG4(double, 1024, 7, 4, 4, 8, 8, 2, 2)
 QSIM_P2CFG(double, 2048, 0, 16, 16, 8, 1, 4)
 QSIM_P2CFG(double, 2048, 1, 16, 16, 8, 2, 2)
 QSIM_P2CFG(double, 2048, 2, 32, 32, 2, 1, 4)
 QSIM_P2CFG(double, 2048, 3, 8, 16, 16, 1, 4)
 QSIM_P2CFG(double, 2048, 4, 16, 16, 8, 1, 8)
 QSIM_P2CFG(double, 2048, 5, 16, 16, 8, 4, 4)
+QSIM_P2CFG4(double, 2048, 6, 8, 8, 8, 4, 1, 4)
+QSIM_P2CFG4(double, 2048, 7, 8, 8, 8, 4, 1, 2)
 QSIM_P2CFG(double, 4096, 0, 16, 16, 16, 0, 0)
 QSIM_P2CFG(double, 8192, 0, 16, 16, 32, 0, 0)  // 1D split jobs only
 QSIM_P2CFG(float, 16, 0, 4, 4, 1, 0, 0)
 QSIM_P2CFG(float, 32, 0, 8, 4, 1, 0, 0)
 QSIM_P2CFG(float, 64, 0, 8, 8, 1, 0, 0)
 QSIM_P2CFG(float, 128, 0, 16, 8, 1, 0, 0)
+QSIM_P2CFG(float, 128, 1, 8, 8, 2, 0, 0)  // 8 per thread (Born sweeps)
 QSIM_P2CFG(float, 256, 0, 16, 16, 1, 0, 0)
 QSIM_P2CFG(float, 512, 0, 16, 32, 1, 0, 0)
 QSIM_P2CFG(float, 1024, 0, 32, 32, 1, 2, 8)
@@ -50,16 +58,21 @@ QSIM_P2CFG(float, 1024, 2, 16, 16, 4, 1, 8)
 QSIM_P2CFG(float, 1024, 3, 16, 16, 4, 2, 16)
 QSIM_P2CFG(float, 1024, 4, 32, 32, 1, 2, 4)
 QSIM_P2CFG(float, 1024, 5, 16, 16, 4, 4, 4)
+QSIM_P2CFG4(float, 1024, 6, 8, 8, 4, 4, 2, 8)
+QSIM_P2CFG4(float, 1024, 7, 8, 8, 4, 4, 1, 4)
 QSIM_P2CFG(float, 2048, 0, 16, 16, 8, 1, 8)
 QSIM_P2CFG(float, 2048, 1, 32, 64, 1, 1, 8)
 QSIM_P2CFG(float, 2048, 2, 64, 32, 1, 1, 8)
 QSIM_P2CFG(float, 2048, 3, 16, 16, 8, 2, 16)
 QSIM_P2CFG(float, 2048, 4, 32, 32, 2, 1, 8)
 QSIM_P2CFG(float, 2048, 5, 16, 16, 8, 1, 4)
+QSIM_P2CFG4(float, 2048, 6, 8, 8, 8, 4, 1, 4)
+QSIM_P2CFG4(float, 2048, 7, 8, 8, 8, 4, 1, 2)
 QSIM_P2CFG(float, 4096, 0, 16, 16, 16, 0, 0)
 QSIM_P2CFG(float, 8192, 0, 16, 16, 32, 0, 0)
 QSIM_P2CFG(float, 16384, 0, 16, 32, 32, 0, 0)  // 1D split jobs only
 #undef QSIM_P2CFG
+#undef QSIM_P2CFG4
 
 __host__ __device__ constexpr int pow2floor(int x) { return x <= 1 ? 1 : 2 * pow2floor(x / 2); }
 

@@ -427,7 +427,7 @@ int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps
 
 // Launch variants: lengths 1024 and 2048 have the candidates 0..kVariants-1
 // (P2Cfg), every other length only variant 0.
-constexpr int kVariants = 6;
+constexpr int kVariants = 8;
 
 bool has_variants(size_t n) { return n == 1024 || n == 2048; }
 
@@ -442,6 +442,8 @@ int row_v(int var, const SplitDev *jobs, int njobs, size_t nrows, int iev, const
             case 3: return launch_row<T, NN, MODE, 3>(jobs, njobs, nrows, iev, tw, s);
             case 4: return launch_row<T, NN, MODE, 4>(jobs, njobs, nrows, iev, tw, s);
             case 5: return launch_row<T, NN, MODE, 5>(jobs, njobs, nrows, iev, tw, s);
+            case 6: return launch_row<T, NN, MODE, 6>(jobs, njobs, nrows, iev, tw, s);
+            case 7: return launch_row<T, NN, MODE, 7>(jobs, njobs, nrows, iev, tw, s);
             default: break;
         }
     }
@@ -459,6 +461,8 @@ int col_v(int var, const SplitDev *jobs, int njobs, size_t ncols, int ik, const 
             case 3: return launch_col<T, NN, 3>(jobs, njobs, ncols, ik, tw, s);
             case 4: return launch_col<T, NN, 4>(jobs, njobs, ncols, ik, tw, s);
             case 5: return launch_col<T, NN, 5>(jobs, njobs, ncols, ik, tw, s);
+            case 6: return launch_col<T, NN, 6>(jobs, njobs, ncols, ik, tw, s);
+            case 7: return launch_col<T, NN, 7>(jobs, njobs, ncols, ik, tw, s);
             default: break;
         }
     }
@@ -931,38 +935,43 @@ int qsim_hip_split_autotune(qsim_split_job *j, int reps, int *row_variant, int *
     QSIM_HIP(hipEventCreate(&a));
     QSIM_HIP(hipEventCreate(&b));
     int rc = QSIM_OK;
-    double best_r = 1e30, best_c = 1e30;
-    int vr = 0, vc = 0;
+    // The sweeps run alternately (each reads what the other wrote, from
+    // other XCDs' caches), so every candidate is timed inside alternating
+    // (row, column) pairs: first the row variant with the column variant
+    // fixed, then the column variant with the chosen row variant.
+    int var[2] = {j->row_var, j->col_var};
+    double best[2] = {1e30, 1e30};
+    auto pair = [&](int vr_, int vc_) {
+        int r2 = row_sweep(prec, j->d.ny, SW_MID, vr_, jt, 1, j->d.nx, j->ph_e[0], j->tw_y, s);
+        if (r2 == QSIM_OK)
+            r2 = col_sweep(prec, j->d.nx, vc_, jt, 1, j->d.ny, j->ph_k[0], j->tw_x, s);
+        return r2;
+    };
     for (int which = 0; which < 2 && rc == QSIM_OK; ++which) {
         const size_t nlen = which == 0 ? j->d.ny : j->d.nx;
         const int nv = has_variants(nlen) ? kVariants : 1;
+        int chosen = var[which];
         for (int v = 0; v < nv && rc == QSIM_OK; ++v) {
-            auto launch = [&]() {
-                return which == 0 ? row_sweep(prec, j->d.ny, SW_MID, v, jt, 1, j->d.nx,
-                                              j->ph_e[0], j->tw_y, s)
-                                  : col_sweep(prec, j->d.nx, v, jt, 1, j->d.ny, j->ph_k[0],
-                                              j->tw_x, s);
-            };
-            for (int w = 0; w < 2 && rc == QSIM_OK; ++w) rc = launch();
+            const int vr_ = which == 0 ? v : var[0], vc_ = which == 1 ? v : var[1];
+            for (int w = 0; w < 2 && rc == QSIM_OK; ++w) rc = pair(vr_, vc_);
             if (rc != QSIM_OK) break;
             (void)hipEventRecord(a, s);
-            for (int r = 0; r < reps && rc == QSIM_OK; ++r) rc = launch();
+            for (int r = 0; r < reps && rc == QSIM_OK; ++r) rc = pair(vr_, vc_);
             (void)hipEventRecord(b, s);
             if (rc != QSIM_OK) break;
             (void)hipEventSynchronize(b);
             float ms = 0;
             (void)hipEventElapsedTime(&ms, a, b);
             const double us = 1e3 * ms / reps;
-            if (which == 0 && us < best_r) {
-                best_r = us;
-                vr = v;
-            }
-            if (which == 1 && us < best_c) {
-                best_c = us;
-                vc = v;
+            if (us < best[which]) {
+                best[which] = us;
+                chosen = v;
             }
         }
+        var[which] = chosen;
     }
+    const int vr = var[0], vc = var[1];
+    const double best_r = best[0], best_c = best[1];
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipStreamSynchronize(s);

@@ -181,3 +181,23 @@ def test_1d_fp32_long_lines(n, oracle):
     err = rel_l2(e.GetPsi(), ref)
     print("1D FP32 n=%d 500 steps rel L2 %.2e" % (n, err))
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("shape", [(1024, 64), (64, 1024), (2048, 64), (64, 2048)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_launch_variants_match_oracle(shape, variant, oracle):
+    """every row / column launch variant of the 1024 and 2048 sweeps (radix
+    plans of three and four passes) against the oracle; FP32-mixed against
+    the FP64 device result"""
+    from qusim_amd._native import call
+    psi, V, L0, L1 = gauss2d(*shape)
+    outs = {}
+    for prec in ("fp64", "mixed32"):
+        e = run_2d(psi, V, L0, L1, 0.05, 0, 2, precision=prec, complex_v=True)
+        call("qsim_hip_split_set_variants", e._job, variant, variant)
+        call("qsim_hip_split_steps", e._job, 6, None)
+        outs[prec] = e.GetPsi()
+    ref = oracle.split2d(psi, V, L0[0], L0[1], L1[0], L1[1], 0.05, 6, method=2, complex_v=True)
+    assert rel_l2(outs["fp64"], ref) < 1e-12
+    assert rel_l2(outs["mixed32"], outs["fp64"]) < 1e-5
