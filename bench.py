@@ -247,6 +247,84 @@ def fp32_section(steps_err, steps_rate):
             "dominant_kernel": dom, "kernels": per}
 
 
+BORN_B_PER_PT = {64: 176.0, 32: 96.0}   # per energy-iteration, 3D fused passes
+
+
+def born_section(max_iter=200, tol=1e-8, checks=(0, 7, 31)):
+    """3D Born series, 128^3 on [-32,32)^3, 32 energies k ~ U(1,3) in one
+    job: complex64 iterates with FP64 V, tables and residual, Vellekoop,
+    slow 0.5, stopping at a relative update below `tol` or `max_iter`.
+    Far-field amplitudes (forward and 37 degrees off axis) of a few energies
+    are compared with complex128 single-energy jobs run for the same number
+    of iterations."""
+    from qusim_amd import _native as Nn
+    from qusim_amd.api import Options, QuPerturbation3D, SolverMethod
+    from qusim_amd.workloads import c4_energies, c4_potential
+    n, L = 128, 32.0
+    V, Es = c4_potential(n=n, L=L), c4_energies()
+
+    def job(prec, energies):
+        o = Options().Hip().Order(1).Preconditional(True).VellekoopPreconditioner().Slow(0.5)
+        if prec == 32:
+            o.Precision("mixed32")
+        return QuPerturbation3D().init(V, -L, L, n, -L, L, n, -L, L, n, energies, 0.0, 0, 0, 1,
+                                       SolverMethod.BornSerise, 1.0, 1.0, o)
+
+    s32 = job(32, Es)
+    Nn.synchronize()
+    t0 = time.perf_counter()
+    s32.Compute(max_iter=max_iter, tol=tol)
+    Nn.synchronize()
+    secs = time.perf_counter() - t0
+    its = s32.iters
+    rate = sum(its) / secs
+    errs = []
+    t64 = 0.0
+    for e in checks:
+        s64 = job(64, Es[e])
+        Nn.synchronize()
+        t1 = time.perf_counter()
+        s64.Compute(max_iter=its[e], tol=0.0)
+        Nn.synchronize()
+        t64 += time.perf_counter() - t1
+        row = {"energy": Es[e], "iterations": its[e]}
+        for tag, d in (("forward", (0.0, 0.0, 1.0)), ("side", (0.6, 0.0, 0.8))):
+            a64, a32 = s64.ScatteringAmplitude(*d), s32.ScatteringAmplitude(*d, e)
+            row["amp_rel_err_" + tag] = abs(a32 - a64) / abs(a64)
+        p64, p32 = s64.GetPsi(), s32.GetPsi(e)
+        row["dpsi_rel_l2"] = float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))
+        errs.append(row)
+        s64.close()
+    s32.close()
+    gbs = rate * n ** 3 * BORN_B_PER_PT[32] / 1e9
+    return {"workload": "3D Born series 128^3 on [-32,32)^3, 32 energies k~U(1,3), Vellekoop "
+                        "slow 0.5, complex64 iterates + FP64 residual, tol %g / %d iterations"
+                        % (tol, max_iter),
+            "value": rate, "unit": "energy-iterations/s", "seconds": secs,
+            "iterations_min_max": [min(its), max(its)],
+            "residual_max": max(s32.residual), "converged": sum(r < tol for r in s32.residual),
+            "fp64_single_energy_rate": sum(errs_i["iterations"] for errs_i in errs) / t64,
+            "algorithmic_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+            "bytes_per_point_iteration": BORN_B_PER_PT[32], "checks_vs_fp64": errs}
+
+
+def serving_section(count, rate, budget_gib, spot_every=50):
+    """Request loop over the seeded mix (60% 1D, 30% 2D evolutions, 10% 2D
+    Born), Poisson arrivals at `rate`/s, FP32-mixed, admission by HBM
+    footprint; every `spot_every`-th request re-solved alone in FP64."""
+    from qusim_amd.serving import Server, make_requests, spot_check
+    reqs = make_requests(count, rate=rate)
+    keep = [r.rid for r in reqs[::spot_every]]
+    srv = Server(precision=32, hbm_budget=budget_gib << 30, keep=keep)
+    st = srv.run(reqs)
+    st["spot_check_fp64"] = spot_check(reqs, srv.results, every=spot_every)
+    st["workload"] = ("%d requests, Poisson %.0f/s: 60%% 1D N 1024..16384 x 100..2000 steps, "
+                      "30%% 2D 256^2..1024^2 x 50..500 steps, 10%% 2D Born 256^2..1024^2 "
+                      "(<= 200 iterations, tol 1e-6); FP32-mixed; HBM budget %d GiB"
+                      % (count, rate, budget_gib))
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +334,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--fp32-err-steps", type=int, default=1000)
+    ap.add_argument("--no-born", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=400.0, help="request arrivals per second")
+    ap.add_argument("--hbm-budget-gib", type=int, default=64)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run the workload of the rocprofv3 --pmc passes and exit")
     ap.add_argument("--probe-steps", type=int, default=6)
@@ -321,6 +404,16 @@ def main():
     if not args.no_fp32:
         log("FP32-mixed 2048^2 section")
         out["fp32_mixed"] = fp32_section(args.fp32_err_steps, 50)
+    if not args.no_born:
+        log("Born 128^3 x 32 energies section")
+        out["born_3d"] = born_section()
+        log("Born: %.0f energy-iterations/s" % out["born_3d"]["value"])
+    if not args.no_serving:
+        log("serving section: %d requests at %.0f/s" % (args.requests, args.rate))
+        out["serving"] = serving_section(args.requests, args.rate, args.hbm_budget_gib)
+        log("serving: %.1f solves/s, p50 %.3f s, p99 %.3f s"
+            % (out["serving"]["solves_per_s"], out["serving"]["latency_p50_s"],
+               out["serving"]["latency_p99_s"]))
     if not args.no_cpu:
         log("CPU baseline: %d steps of the oracle on 1 thread" % args.cpu_steps)
         rate, secs = cpu_baseline(prob, args.cpu_steps)

@@ -126,7 +126,7 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *slots)
 template <typename T, int N, int MODE, int PRE>
 __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
     k_born_inner(const BornDev *__restrict__ jobs, const c128 *__restrict__ V, BornGeom geo,
-                 const cx<T> *__restrict__ tw)
+                 const c128 *__restrict__ tw)
 {
     using LP = P2Launch<T, N, false, BornV<N>::V>;
     using P = typename LP::Line;
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
 // Outer sweep: lines along axis 0, stride S = n1*n2, G0 between FFT and IFFT.
 template <typename T, int N>
 __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
-    k_born_outer(const BornDev *__restrict__ jobs, BornGeom geo, const cx<T> *__restrict__ tw)
+    k_born_outer(const BornDev *__restrict__ jobs, BornGeom geo, const c128 *__restrict__ tw)
 {
     using LP = P2Launch<T, N, true>;
     using P = typename LP::Line;
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
 // Middle axis (3D y): plain transform of lines [n0][n1][n2] along axis 1.
 template <typename T, int N, bool INV>
 __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
-    k_born_mid(const BornDev *__restrict__ jobs, BornGeom geo, const cx<T> *__restrict__ tw)
+    k_born_mid(const BornDev *__restrict__ jobs, BornGeom geo, const c128 *__restrict__ tw)
 {
     using LP = P2Launch<T, N, true>;
     using P = typename LP::Line;
@@ -421,7 +421,7 @@ int launch_inner_t(const BornDev *tab, int nact, const c128 *V, const BornGeom &
     size_t nlines = geo.n0 * geo.nmid;
     dim3 grid((unsigned)((nlines + LP::L - 1) / LP::L), (unsigned)nact);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, V, geo,
-                       (const cx<T> *)tw);
+                       (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
 }
@@ -485,7 +485,7 @@ int launch_outer_t(const BornDev *tab, int nact, const BornGeom &geo, const void
                                      (int)LP::LDS_BYTES));
     size_t S = geo.n1 * geo.n2;
     dim3 grid((unsigned)((S + LP::L - 1) / LP::L), (unsigned)nact);
-    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, geo, (const cx<T> *)tw);
+    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, geo, (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
 }
@@ -509,7 +509,7 @@ int launch_mid_t(const BornDev *tab, int nact, const BornGeom &geo, const void *
                                      (int)LP::LDS_BYTES));
     size_t nlines = geo.n0 * geo.n2;
     dim3 grid((unsigned)((nlines + LP::L - 1) / LP::L), (unsigned)nact);
-    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, geo, (const cx<T> *)tw);
+    hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, geo, (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
 }

@@ -15,16 +15,18 @@ namespace qsim {
 
 // ------------------------------------------------------------ twiddle tables
 // tw[m] = exp(-2 pi i m / n), m < n, from x87 long double, rounded once to
-// double; FP32 tables hold the float high parts, then the float low parts.
+// double.  Kernels of both precisions read this FP64 table (FP32 kernels
+// split each power into a float-float pair in registers).
 static std::mutex g_tw_mu;
-static std::map<std::pair<int, std::pair<size_t, int>>, void *> g_tw;
+static std::map<std::pair<int, size_t>, void *> g_tw;
 
 int twiddle_table(size_t n, int precision, const void **out)
 {
+    (void)precision;  // one FP64 table per length serves both precisions
     int dev = 0;
     QSIM_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_tw_mu);
-    auto key = std::make_pair(dev, std::make_pair(n, precision));
+    auto key = std::make_pair(dev, n);
     auto it = g_tw.find(key);
     if (it != g_tw.end()) {
         *out = it->second;
@@ -38,19 +40,8 @@ int twiddle_table(size_t n, int precision, const void **out)
         h[2 * m + 1] = (double)sinl(a);
     }
     void *d = nullptr;
-    if (precision == 64) {
-        QSIM_HIP(hipMalloc(&d, n * sizeof(c128)));
-        QSIM_HIP(hipMemcpy(d, h.data(), n * sizeof(c128), hipMemcpyHostToDevice));
-    } else {
-        // n float high parts, then n float low parts (double - float)
-        std::vector<float> f(4 * n);
-        for (size_t i = 0; i < 2 * n; ++i) {
-            f[i] = (float)h[i];
-            f[2 * n + i] = (float)(h[i] - (double)f[i]);
-        }
-        QSIM_HIP(hipMalloc(&d, 2 * n * sizeof(c64)));
-        QSIM_HIP(hipMemcpy(d, f.data(), 2 * n * sizeof(c64), hipMemcpyHostToDevice));
-    }
+    QSIM_HIP(hipMalloc(&d, n * sizeof(c128)));
+    QSIM_HIP(hipMemcpy(d, h.data(), n * sizeof(c128), hipMemcpyHostToDevice));
     g_tw[key] = d;
     *out = d;
     return QSIM_OK;
@@ -77,7 +68,7 @@ static int launch_p2_inst(const void *in, void *out, size_t S, size_t nlines, co
     attr_done = true;
     size_t blocks = (nlines + LP::L - 1) / LP::L;
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(LP::THREADS), LP::LDS_BYTES, s,
-                       (const cx<T> *)in, (cx<T> *)out, S, nlines, (const cx<T> *)tw, (T)scale);
+                       (const cx<T> *)in, (cx<T> *)out, S, nlines, (const c128 *)tw, (T)scale);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
 }
@@ -159,8 +150,8 @@ static int launch_4s_inst(const void *in, void *out, size_t S, size_t nlines, si
                                      (int)LP::LDS_BYTES));
     size_t blocks = (nlines + LP::L - 1) / LP::L;
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(LP::THREADS), LP::LDS_BYTES, s,
-                       (const cx<T> *)in, (cx<T> *)out, S, nlines, other, (const cx<T> *)tw,
-                       (const cx<T> *)twn, nbig, (T)scale);
+                       (const cx<T> *)in, (cx<T> *)out, S, nlines, other, (const c128 *)tw,
+                       (const c128 *)twn, nbig, (T)scale);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
 }
@@ -290,7 +281,7 @@ int run_axis(const FftAxis &a, int precision, bool inv, const void *in, void *ou
         }
         hipLaunchKernelGGL(k_fft_generic<float>, dim3((unsigned)blocks), dim3(256), a.lds, s,
                            (const c64 *)in, (c64 *)out, a.S, a.nlines, a.L, a.gf,
-                           (const c64 *)a.tw, (int)inv, (float)scale);
+                           (const c128 *)a.tw, (int)inv, (float)scale);
     }
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;

@@ -43,6 +43,12 @@ __device__ __forceinline__ c64 mul_ff(c64 a, c64 h, c64 l)
     const float sr = a.x * l.x - a.y * l.y, si = a.x * l.y + a.y * l.x;
     return mk(fmaf(a.x, h.x, fmaf(-a.y, h.y, sr)), fmaf(a.x, h.y, fmaf(a.y, h.x, si)));
 }
+// float-float pair of an FP64 complex: hi = nearest float, lo = the rest
+__device__ __forceinline__ void split_ff(c128 w, c64 &h, c64 &l)
+{
+    h = mk((float)w.x, (float)w.y);
+    l = mk((float)(w.x - (double)h.x), (float)(w.y - (double)h.y));
+}
 // a * conj(h + l)
 __device__ __forceinline__ c64 mulconj_ff(c64 a, c64 h, c64 l)
 {
@@ -166,12 +172,12 @@ struct Pow2Line {
         return t + u * TPL + r * (N / R);
     }
 
-    // twiddle + butterfly for a pass of radix R after NS points combined.
-    // FP64: one table load per butterfly, the powers w^r by multiplication
-    // (error ~r ulp, far below the 1e-12 bar); FP32: every power from the
-    // table as a float-float pair.
+    // twiddle + butterfly for a pass of radix R after NS points combined:
+    // one FP64 table load per butterfly, the powers w^r by FP64
+    // multiplication (error ~r ulp, far below the 1e-12 bar); FP32 data
+    // takes each power as a float-float pair.
     template <int R, int NS, bool INV>
-    static __device__ __forceinline__ void pass(cx<T> *v, int t, const cx<T> *__restrict__ tw)
+    static __device__ __forceinline__ void pass(cx<T> *v, int t, const c128 *__restrict__ tw)
     {
 #pragma unroll
         for (int u = 0; u < E / R; ++u) {
@@ -188,12 +194,18 @@ struct Pow2Line {
                         if (r + 1 < R) w = w * w1;
                     }
                 } else {
-                    // FP32 tables hold n high parts, then n low parts
+                    // the same FP64 powers, each split into a float-float
+                    // pair (one table load per butterfly, no rounded
+                    // constant applied twice)
+                    const c128 w1 = tw[step];
+                    c128 w = w1;
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
-                        const cx<T> wh = tw[step * r], wl = tw[N + step * r];
+                        c64 wh, wl;
+                        split_ff(w, wh, wl);
                         v[u * R + r] = INV ? mulconj_ff(v[u * R + r], wh, wl)
                                            : mul_ff(v[u * R + r], wh, wl);
+                        if (r + 1 < R) w = w * w1;
                     }
                 }
             }
@@ -224,7 +236,7 @@ struct Pow2Line {
     // Forward transform.  In: inputs of radix-RA pass (in_pos<RA>).  Out: X at
     // in_pos<RLAST>(t, u, r) with RLAST = RD, RC, RB or RA (NP = 4, 3, 2, 1).
     static __device__ __forceinline__ void forward(cx<T> *v, int t, cx<T> *lds,
-                                                   const cx<T> *__restrict__ tw)
+                                                   const c128 *__restrict__ tw)
     {
         pass<RA, 1, false>(v, t, tw);
         if constexpr (NP >= 2) {
@@ -246,7 +258,7 @@ struct Pow2Line {
     // Inverse transform, radices RLAST, ..., RA.  In: in_pos<RLAST>.  Out: x at
     // in_pos<RA>(t, u, r).
     static __device__ __forceinline__ void inverse(cx<T> *v, int t, cx<T> *lds,
-                                                   const cx<T> *__restrict__ tw)
+                                                   const c128 *__restrict__ tw)
     {
         if constexpr (NP == 1) {
             pass<RA, 1, true>(v, t, tw);

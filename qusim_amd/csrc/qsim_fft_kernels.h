@@ -99,7 +99,7 @@ struct P2Launch {
 template <typename T, int N, bool LF, bool INV>
 __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
     k_fft_p2(const cx<T> *in, cx<T> *out, size_t S, size_t nlines,
-             const cx<T> *__restrict__ tw, T scale)
+             const c128 *__restrict__ tw, T scale)
 {
     using LP = P2Launch<T, N, LF>;
     using P = typename LP::Line;
@@ -145,7 +145,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
 template <typename T, int N, bool LF, bool INV, int PASS>
 __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
     k_fft_4step(const cx<T> *in, cx<T> *out, size_t S, size_t nlines, size_t other,
-                const cx<T> *__restrict__ tw, const cx<T> *__restrict__ twn, size_t nbig,
+                const c128 *__restrict__ tw, const c128 *__restrict__ twn, size_t nbig,
                 T scale)
 {
     using LP = P2Launch<T, N, LF>;
@@ -190,7 +190,9 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
                 if constexpr (sizeof(T) == 8) {
                     a = a * twn[m];
                 } else {
-                    a = mul_ff(a, twn[m], twn[nbig + m]);
+                    c64 h, lo;
+                    split_ff(twn[m], h, lo);
+                    a = mul_ff(a, h, lo);
                 }
                 if constexpr (INV) a.y = -a.y;
                 out[base + (size_t)k2 * SI] = a;
@@ -233,7 +235,7 @@ struct GenFactors {
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_fft_generic(const cx<T> *in, cx<T> *out, size_t S, size_t nlines, int L, GenFactors gf,
-                  const cx<T> *__restrict__ tw, int inverse, T scale)
+                  const c128 *__restrict__ tw, int inverse, T scale)
 {
     extern __shared__ unsigned char smem_raw[];
     const int N = gf.n;
@@ -271,7 +273,7 @@ __global__ void __launch_bounds__(256)
             for (int q = 1; q < R; ++q) {
                 e += mstep;
                 if (e >= N) e -= N;
-                cx<T> w = tw[e];
+                const cx<T> w = cvt<T>(tw[e]);
                 cx<T> x = s[q * NR];
                 acc = acc + (inverse ? mulconj(x, w) : x * w);
             }

@@ -61,7 +61,7 @@ __device__ __forceinline__ cx<TO> conv(cx<TI> a)
 template <typename T, int N, int MODE, int V>
 __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
     k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
-                const cx<T> *__restrict__ tw)
+                const c128 *__restrict__ tw)
 {
     using LP = P2Launch<T, N, false, V>;
     using P = typename LP::Line;
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
     k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
-                const cx<T> *__restrict__ tw)
+                const c128 *__restrict__ tw)
 {
     using LP = P2Launch<T, N, true, V>;
     using P = typename LP::Line;
@@ -198,7 +198,7 @@ struct Prog1D {
 template <typename T, int N>
 __global__ void __launch_bounds__((P2Cfg<T, N>::Line::TPL))
     k_split1d(const SplitDev *__restrict__ jobs, Prog1D prog, size_t nsteps,
-              const cx<T> *__restrict__ tw)
+              const c128 *__restrict__ tw)
 {
     using P = typename P2Cfg<T, N>::Line;
     extern __shared__ unsigned char smem_raw[];
@@ -370,7 +370,7 @@ int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const voi
                                      (int)LP::LDS_BYTES));
     dim3 grid((unsigned)((nrows + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, nrows, iev,
-                       (const cx<T> *)tw);
+                       (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
 }
@@ -386,7 +386,7 @@ int launch_col(const SplitDev *jobs, int njobs, size_t ncols, int ik, const void
                                      (int)LP::LDS_BYTES));
     dim3 grid((unsigned)((ncols + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, ncols, ik,
-                       (const cx<T> *)tw);
+                       (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
 }
@@ -402,7 +402,7 @@ int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps
         QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds));
     hipLaunchKernelGGL(fn, dim3((unsigned)njobs), dim3(P::TPL), lds, s, jobs, prog, nsteps,
-                       (const cx<T> *)tw);
+                       (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
 }

@@ -116,8 +116,15 @@ def footprint(req, precision):
         return pts * (32 if precision == 32 else 48) + (1 << 20)
     if precision == 32:
         return pts * 72 + (1 << 20)
-    fused = req.n <= (8192 if req.kind == "evo1d" else 4096)
-    return pts * (64 if fused else 192) + (1 << 20)
+    return pts * (64 if fused(req, precision) else 192) + (1 << 20)
+
+
+def fused(req, precision):
+    """whether the split job runs the fused kernels (batchable); FP64 lines
+    beyond the fused lengths take the unfused order on the four-step FFT"""
+    if req.kind == "born2d" or precision == 32:
+        return True
+    return req.n <= (8192 if req.kind == "evo1d" else 4096)
 
 
 def default_quantum(req):
@@ -189,6 +196,7 @@ class EvoJob:
         N.call("qsim_hip_split_create", ctypes.byref(h), ctypes.byref(d), vre.ptr, vim.ptr,
                self.psi.ptr)
         self.job = h.value
+        self.fused = fused(req, precision)
         self.done = 0
         self.dv = ((p["hi"] - p["lo"]) / req.n) ** (1 if req.kind == "evo1d" else 2)
 
@@ -306,8 +314,12 @@ class _Lane:
             groups.setdefault(j.next_chunk(self.quantum), []).append(j)
         self.inflight = []
         for chunk, js in groups.items():
-            for k in range(0, len(js), self.CAPACITY):
-                part = js[k:k + self.CAPACITY]
+            batched = [j for j in js if j.fused]
+            for j in js:
+                if not j.fused:  # unfused order: one job at a time, same stream
+                    N.call("qsim_hip_split_steps", j.job, chunk, self.stream)
+            for k in range(0, len(batched), self.CAPACITY):
+                part = batched[k:k + self.CAPACITY]
                 arr = (ctypes.c_void_p * len(part))(*[j.job for j in part])
                 N.call("qsim_hip_split_batch_steps", self.batch, arr, len(part), chunk,
                        self.stream)
@@ -535,9 +547,14 @@ class Server:
         }
 
 
-def solve_alone(req, precision=32, keep=True):
-    """One request served by itself (arrival at t = 0): its result record."""
-    r = Request(req.rid, req.kind, req.n, req.work, 0.0, req.params)
+def solve_alone(req, precision=32, keep=True, iterations=None):
+    """One request served by itself (arrival at t = 0): its result record.
+    iterations: run a Born solve for exactly that many iterations (no
+    tolerance stop), e.g. to match another precision's run."""
+    params, work = req.params, req.work
+    if iterations is not None:
+        params, work = dict(req.params, tol=0.0), iterations
+    r = Request(req.rid, req.kind, req.n, work, 0.0, params)
     s = Server(precision=precision, realtime=False, keep=(req.rid,) if keep else ())
     s.run([r])
     if req.rid in s.errors:
@@ -560,7 +577,11 @@ def spot_check(requests, results, every=50):
     for r in requests[::every]:
         if r.rid not in results:
             continue
-        ref = solve_alone(r, precision=64)
-        errs[r.kind].append(compare(r.kind, results[r.rid], ref))
+        got = results[r.rid]
+        # a Born twin runs as many iterations as the FP32 solve did, so the
+        # difference is the precision's, not the stopping point's
+        its = got["iterations"] if r.kind == "born2d" else None
+        ref = solve_alone(r, precision=64, iterations=its)
+        errs[r.kind].append(compare(r.kind, got, ref))
     return {k: {"checked": len(v), "max": max(v) if v else None,
                 "median": float(np.median(v)) if v else None} for k, v in errs.items()}
