@@ -165,7 +165,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
             vv[u * RA + r] = active ? V[base + k] : mk(0.0, 0.0);
             dd[u * RA + r] = active ? dp[base + k] : mk<T>(0, 0);
         }
-    if constexpr (MODE != BA_START) P::inverse(v, t, lds + l * P::LS, tw);
+    if constexpr (MODE != BA_START) P::inverse(v, t, lds + l * LP::LSTR, tw);
     // per-line factors of the separable tables; the inner index varies below
     const double ab_line = dim == 3 ? J.ab[0][i] + J.ab[1][j] : J.ab[0][i];
     const c128 pw_line = dim == 3 ? J.pw[0][i] * J.pw[1][j] : J.pw[0][i];
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
             }
         }
     if constexpr (MODE != BA_END) {
-        P::forward(v, t, lds + l * P::LS, tw);
+        P::forward(v, t, lds + l * LP::LSTR, tw);
         if (active) {
 #pragma unroll
             for (int u = 0; u < P::E / RZ; ++u)
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
         for (int r = 0; r < RA; ++r)
             v[u * RA + r] = active ? tmp[c + (size_t)P::template in_pos<RA>(t, u, r) * S]
                                    : mk<T>(0, 0);
-    P::forward(v, t, lds + l * P::LS, tw);
+    P::forward(v, t, lds + l * LP::LSTR, tw);
 #pragma unroll
     for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
@@ -261,7 +261,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
             const c128 z = cdiv(cvt<double>(v[u * RZ + r]), mk(J.E - tt, J.eps));
             v[u * RZ + r] = cvt<T>(z);
         }
-    P::inverse(v, t, lds + l * P::LS, tw);
+    P::inverse(v, t, lds + l * LP::LSTR, tw);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / RA; ++u)
@@ -300,9 +300,9 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
             v[u * R0 + r] = active ? tmp[base + (size_t)P::template in_pos<R0>(t, u, r) * S]
                                    : mk<T>(0, 0);
     if constexpr (INV)
-        P::inverse(v, t, lds + l * P::LS, tw);
+        P::inverse(v, t, lds + l * LP::LSTR, tw);
     else
-        P::forward(v, t, lds + l * P::LS, tw);
+        P::forward(v, t, lds + l * LP::LSTR, tw);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / R1; ++u)

@@ -86,13 +86,21 @@ struct P2Launch {
     static constexpr int LF_LINES = 128 / (int)sizeof(cx<T>);
     static constexpr int ROWS = cmax(1, 256 / Line::TPL);
     static constexpr int L_WANT = LF ? LF_LINES : ROWS;
-    static constexpr int L_LDS = cmax(1, (96 * 1024) / (Line::LS * (int)sizeof(cx<T>)));
+    static constexpr int LS_MAX = Line::LS_RAW + 15;
+    static constexpr int L_LDS = cmax(1, (96 * 1024) / (LS_MAX * (int)sizeof(cx<T>)));
     static constexpr int L_FIXED = LF ? Cfg::COL_L : Cfg::ROW_L;
     // never more lines than the 160 KiB of LDS one workgroup can hold
-    static constexpr int L_CAP = pow2floor(cmax(1, (160 * 1024) / (Line::LS * (int)sizeof(cx<T>))));
+    static constexpr int L_CAP = pow2floor(cmax(1, (160 * 1024) / (LS_MAX * (int)sizeof(cx<T>))));
     static constexpr int L = cmin(L_CAP, L_FIXED > 0 ? L_FIXED : pow2floor(cmin(L_WANT, L_LDS)));
     static constexpr int THREADS = L * Line::TPL;
-    static constexpr size_t LDS_BYTES = (size_t)L * Line::LS * sizeof(cx<T>);
+    // LDS stride between the lines of a workgroup (in elements).  Line-fast
+    // tiles put lane i on line i % L, element i / L: a stride of 16/L (mod 16)
+    // sends the lanes of one 16-lane group to 16 different 16-byte slots;
+    // rows keep 1 (mod 16).
+    static constexpr int TARGET = LF ? (L >= 16 ? 1 : 16 / L) : 1;
+    static constexpr int LSTR =
+        Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
+    static constexpr size_t LDS_BYTES = (size_t)L * LSTR * sizeof(cx<T>);
 };
 
 // Plain transform of `nlines` lines of length N with stride S (in may be out).
@@ -122,9 +130,9 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
             v[u * R0 + r] = active ? in[base + (size_t)P::template in_pos<R0>(t, u, r) * S]
                                    : mk<T>(0, 0);
     if constexpr (INV)
-        P::inverse(v, t, lds + l * P::LS, tw);
+        P::inverse(v, t, lds + l * LP::LSTR, tw);
     else
-        P::forward(v, t, lds + l * P::LS, tw);
+        P::forward(v, t, lds + l * LP::LSTR, tw);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / R1; ++u)
@@ -175,7 +183,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
 #pragma unroll
         for (int e = 0; e < P::E; ++e) v[e].y = -v[e].y;
     }
-    P::forward(v, t, lds + l * P::LS, tw);
+    P::forward(v, t, lds + l * LP::LSTR, tw);
     constexpr int RZ = P::RLAST;
     if (!active) return;
     if constexpr (PASS == 0) {

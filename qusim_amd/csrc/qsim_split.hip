@@ -106,7 +106,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
                     evp[u * RA + r] = active ? ev[base + P::template in_pos<RA>(t, u, r)]
                                              : mk(0.0, 0.0);
         }
-        P::inverse(v, t, lds + l * P::LS, tw);
+        P::inverse(v, t, lds + l * LP::LSTR, tw);
 #pragma unroll
         for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
@@ -129,7 +129,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
                              conv<double>(v[u * RA + r]));
         }
     } else {
-        P::forward(v, t, lds + l * P::LS, tw);
+        P::forward(v, t, lds + l * LP::LSTR, tw);
         if (active) {
 #pragma unroll
             for (int u = 0; u < P::E / RZ; ++u)
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
         for (int r = 0; r < RA; ++r)
             v[u * RA + r] =
                 active ? work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols] : mk<T>(0, 0);
-    P::forward(v, t, lds + l * P::LS, tw);
+    P::forward(v, t, lds + l * LP::LSTR, tw);
 #pragma unroll
     for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
             const int k = P::template in_pos<RZ>(t, u, r);
             v[u * RZ + r] = apply(v[u * RZ + r], kx[k] * kyj);
         }
-    P::inverse(v, t, lds + l * P::LS, tw);
+    P::inverse(v, t, lds + l * LP::LSTR, tw);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / RA; ++u)
