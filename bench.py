@@ -124,7 +124,7 @@ def traffic_from_profiles(name):
 
 # kernels the PMC summary reports: (key, substring of the rocprof kernel name)
 PROBE_COPY_N = 1 << 20          # complex128 elements per calibration copy (16 MiB each way)
-PROBE_VARIANTS = (4, 4)         # row / column launch variants of the traffic probe
+PROBE_VARIANTS = (6, 4)         # row / column launch variants of the traffic probe
 
 
 def probe_kernels(rv, cv):
@@ -308,15 +308,22 @@ def born_section(max_iter=200, tol=1e-8, checks=(0, 7, 31)):
             "bytes_per_point_iteration": BORN_B_PER_PT[32], "checks_vs_fp64": errs}
 
 
-def serving_section(count, rate, budget_gib, spot_every=50):
+def serving_section(count, rate, budget_gib, spot_every=50, capacity_count=600):
     """Request loop over the seeded mix (60% 1D, 30% 2D evolutions, 10% 2D
     Born), Poisson arrivals at `rate`/s, FP32-mixed, admission by HBM
-    footprint; every `spot_every`-th request re-solved alone in FP64."""
+    footprint; every `spot_every`-th request re-solved alone in FP64.  The
+    capacity line serves the first `capacity_count` requests all arriving at
+    t = 0 (solves/s with the device never idle)."""
     from qusim_amd.serving import Server, make_requests, spot_check
     reqs = make_requests(count, rate=rate)
+    cap = Server(precision=32, hbm_budget=budget_gib << 30, realtime=False)
+    cst = cap.run(reqs[:capacity_count])
     keep = [r.rid for r in reqs[::spot_every]]
     srv = Server(precision=32, hbm_budget=budget_gib << 30, keep=keep)
     st = srv.run(reqs)
+    st["capacity"] = {"requests": cst["requests"], "solves_per_s": cst["solves_per_s"],
+                      "makespan_s": cst["makespan_s"]}
+    st["offered_rate_per_s"] = rate
     st["spot_check_fp64"] = spot_check(reqs, srv.results, every=spot_every)
     st["workload"] = ("%d requests, Poisson %.0f/s: 60%% 1D N 1024..16384 x 100..2000 steps, "
                       "30%% 2D 256^2..1024^2 x 50..500 steps, 10%% 2D Born 256^2..1024^2 "
@@ -337,7 +344,7 @@ def main():
     ap.add_argument("--no-born", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=2000)
-    ap.add_argument("--rate", type=float, default=400.0, help="request arrivals per second")
+    ap.add_argument("--rate", type=float, default=200.0, help="request arrivals per second")
     ap.add_argument("--hbm-budget-gib", type=int, default=64)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run the workload of the rocprofv3 --pmc passes and exit")

@@ -26,6 +26,7 @@
 
 #include "qsim_fft_kernels.h"
 #include "qsim_fft_plan.h"
+#include "qsim_reduce.h"
 
 namespace qsim {
 
@@ -532,17 +533,15 @@ void destroy(qsim_born_job *j)
     delete j;
 }
 
-// epsilon rule (src/Perburbation.h:329-344) over V_eff of energy e
+// epsilon rule (src/Perburbation.h:329-344) over V_eff of energy e, reduced
+// on the fly (V_eff = V + i absorbing layer from the per-axis tables)
 int epsilon_for(qsim_born_job *j, int e, double *eps)
 {
     const BornDev J = j->host[e];
     const BornGeom geo = j->geo;
     const c128 *V = j->V;
     const size_t n = j->n;
-    void *p;
-    QSIM_HIP(hipMalloc(&p, n * sizeof(c128)));
-    c128 *ve = (c128 *)p;
-    int rc = each_b(n, nullptr, [=] __device__(size_t idx) {
+    auto ve_at = [=] __device__(size_t idx) {
         size_t i, jj, k;
         if (geo.dim == 3) {
             k = idx % geo.n2;
@@ -553,21 +552,23 @@ int epsilon_for(qsim_born_job *j, int e, double *eps)
             jj = idx % geo.n1;
             i = idx / geo.n1;
         }
-        ve[idx] = v_eff(V[idx], J, geo.dim, i, jj, k);
-    });
+        return v_eff(V[idx], J, geo.dim, i, jj, k);
+    };
     double minEps = 0;
-    if (rc == QSIM_OK) {
-        if (j->pre == PRE_IDENTITY) {
-            double A = 0, A2 = 0;
-            rc = qsim_hip_sum_imag((const qsim_c128 *)ve, n, &A, nullptr);
-            if (rc == QSIM_OK) rc = qsim_hip_norm2((const qsim_c128 *)ve, n, &A2, nullptr);
-            minEps = 0.5 * A2 / (-A);
-        } else {
-            rc = qsim_hip_max_abs((const qsim_c128 *)ve, n, &minEps, nullptr);
-        }
+    if (j->pre == PRE_IDENTITY) {
+        double r[2];
+        QSIM_TRY(red::reduce_sum<2>(n, nullptr, r, [=] __device__(size_t idx, double *v) {
+            const c128 ve = ve_at(idx);
+            v[0] = ve.y;
+            v[1] = abs2(ve);
+        }));
+        minEps = 0.5 * r[1] / (-r[0]);
+    } else {
+        double m2 = 0;
+        QSIM_TRY(red::reduce_ext<true>(n, nullptr, 0.0, &m2,
+                                       [=] __device__(size_t idx) { return abs2(ve_at(idx)); }));
+        minEps = std::sqrt(m2);
     }
-    (void)hipFree(p);
-    if (rc != QSIM_OK) return rc;
     if (*eps < minEps) *eps = minEps;
     return QSIM_OK;
 }
@@ -958,49 +959,51 @@ int qsim_hip_born_xsection(qsim_born_job *j, int e, double cosx, double cosy, do
     const double k0 = std::sqrt(2 * En * d.mass) / d.hbar;
     double dx[3] = {1, 1, 1};
     for (int a = 0; a < d.dim; ++a) dx[a] = (d.hi[a] - d.lo[a]) / d.n[a];
-    // psi0 + dpsi as complex128 in a scratch array, then the device reduction
-    void *p;
-    QSIM_HIP(hipMalloc(&p, j->n * sizeof(c128)));
-    c128 *tot = (c128 *)p;
+    // r = sum (psi0 + dpsi) Re V e^{-i k.x}: the Xsection3D / 2D host-loop
+    // sums (src/DeviceTemplate.h:1646-1665, src/ScatteringSolverImpl.cpp:
+    // 186-191) with psi0 taken from the separable tables
     const BornDev J = j->host[e];
     const int prec = j->prec;
+    const c128 *V = j->V;
     hipStream_t s = as_stream(stream);
-    int rc = each_b(j->n, s, [=] __device__(size_t idx) {
-        size_t i, jj, k;
-        if (geo.dim == 3) {
-            k = idx % geo.n2;
-            jj = (idx / geo.n2) % geo.n1;
-            i = idx / (geo.n1 * geo.n2);
-        } else {
-            k = 0;
-            jj = idx % geo.n1;
-            i = idx / geo.n1;
-        }
-        c128 dp = prec == 64 ? ((const c128 *)J.dpsi)[idx] : cvt<double>(((const c64 *)J.dpsi)[idx]);
-        tot[idx] = psi0_at(J, geo.dim, i, jj, k) + dp;
-    });
-    qsim_c128 r{0, 0};
-    // a zero psi0 argument: tot already holds psi0 + dpsi
-    void *zp = nullptr;
-    if (rc == QSIM_OK) rc = hipMalloc(&zp, j->n * sizeof(c128)) == hipSuccess ? QSIM_OK : QSIM_ERR_NOMEM;
-    if (rc == QSIM_OK) rc = hipMemsetAsync(zp, 0, j->n * sizeof(c128), s) == hipSuccess ? QSIM_OK : QSIM_ERR_HIP;
-    if (rc == QSIM_OK) {
-        if (d.dim == 3) {
-            double nm = 1 / std::sqrt(cosx * cosx + cosy * cosy + cosz * cosz);
-            rc = qsim_hip_xsection3d((const qsim_c128 *)zp, (const qsim_c128 *)tot,
-                                     (const qsim_c128 *)j->V, k0 * cosx * nm, k0 * cosy * nm,
-                                     k0 * cosz * nm, d.lo[0], d.lo[1], d.lo[2], dx[0], dx[1], dx[2],
-                                     d.n[0], d.n[1], d.n[2], &r, stream);
-        } else {
-            double nm = 1 / std::sqrt(cosx * cosx + cosy * cosy);
-            rc = qsim_hip_xsection2d((const qsim_c128 *)zp, (const qsim_c128 *)tot,
-                                     (const qsim_c128 *)j->V, k0 * cosx * nm, k0 * cosy * nm,
-                                     d.lo[0], d.lo[1], dx[0], dx[1], d.n[0], d.n[1], &r, stream);
-        }
+    const double x0 = d.lo[0], y0 = d.lo[1], z0 = d.lo[2];
+    const double ddx = dx[0], ddy = dx[1], ddz = dx[2];
+    auto tot_at = [=] __device__(size_t idx, size_t i, size_t jj, size_t k) {
+        const c128 dp = prec == 64 ? ((const c128 *)J.dpsi)[idx]
+                                   : cvt<double>(((const c64 *)J.dpsi)[idx]);
+        return psi0_at(J, geo.dim, i, jj, k) + dp;
+    };
+    double rs[2];
+    if (d.dim == 3) {
+        const double nm = 1 / std::sqrt(cosx * cosx + cosy * cosy + cosz * cosz);
+        const double kx = k0 * cosx * nm, ky = k0 * cosy * nm, kz = k0 * cosz * nm;
+        const size_t ny = geo.n1, nz = geo.n2;
+        QSIM_TRY(red::reduce_sum<2>(j->n, s, rs, [=] __device__(size_t idx, double *v) {
+            const size_t iz = idx % nz, q = idx / nz, iy = q % ny, ix = q / ny;
+            const double ph = (x0 + (double)ix * ddx) * kx + (y0 + (double)iy * ddy) * ky +
+                              (z0 + (double)iz * ddz) * kz;
+            double sn, cs;
+            sincos(-ph, &sn, &cs);
+            const c128 t = (tot_at(idx, ix, iy, iz) * V[idx].x) * mk(cs, sn);
+            v[0] = t.x;
+            v[1] = t.y;
+        }));
+    } else {
+        const double nm = 1 / std::sqrt(cosx * cosx + cosy * cosy);
+        const double kx = k0 * cosx * nm, ky = k0 * cosy * nm;
+        const size_t ny = geo.n1;
+        QSIM_TRY(red::reduce_sum<2>(j->n, s, rs, [=] __device__(size_t idx, double *v) {
+            const size_t ix = idx / ny, iy = idx % ny;
+            const double x = x0 + ddx * (double)ix, y = y0 + ddy * (double)iy;
+            const double ph = kx * x + ky * y;
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            const c128 t = (tot_at(idx, ix, iy, 0) * V[idx].x) * mk(cs, -sn);
+            v[0] = t.x;
+            v[1] = t.y;
+        }));
     }
-    (void)hipFree(p);
-    if (zp) (void)hipFree(zp);
-    if (rc != QSIM_OK) return rc;
+    const qsim_c128 r{rs[0], rs[1]};
     const cd rr(r.re, r.im);
     double ds;
     if (d.dim == 3) {

@@ -516,7 +516,9 @@ struct SweepTimer {
     int mark(int k, hipStream_t s)
     {
         hipEvent_t e;
-        QSIM_HIP(hipEventCreate(&e));
+        // timing-only events: no system-scope fence (cache writeback and
+        // invalidate) between the kernels they bracket
+        QSIM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         ev.push_back(e);
         if (k >= 0) kind.push_back(k);
         QSIM_HIP(hipEventRecord(e, s));

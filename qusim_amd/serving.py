@@ -13,8 +13,8 @@ the device at once instead:
   (qsim_hip_split_batch_steps) on its own stream, chunk by chunk.  New jobs
   join the lane between chunks, and finished jobs leave it, while the other
   lanes keep running.
-* Born solves run on a worker thread with a stream of their own; each solve
-  advances a fixed number of iterations per turn.
+* Born solves run on worker threads, each with a stream of its own; each
+  solve advances a fixed number of iterations per turn.
 * A job advances in chunks of a fixed length set by its kind and size
   (`quantum`), never by what else is on the device.  The batched kernels
   compute every job independently.  So a result is bitwise the same alone
@@ -359,12 +359,13 @@ class Server:
     (psi / dpsi) are returned."""
 
     def __init__(self, precision=32, hbm_budget=32 << 30, realtime=True, keep=(),
-                 quantum=default_quantum):
+                 quantum=default_quantum, born_workers=4):
         self.precision = precision
         self.budget = int(hbm_budget)
         self.realtime = realtime
         self.keep = set(keep)
         self.quantum = quantum
+        self.born_workers = int(born_workers)
         self._lock = threading.Lock()
         self._used = 0
         self.peak_used = 0
@@ -372,6 +373,9 @@ class Server:
         self.results = {}
         self.errors = {}
         self._ndone = 0
+        # finished jobs not yet freed: a free waits for the whole device, so
+        # it happens when the lanes are idle or memory is needed
+        self._graveyard = []
 
     # footprint accounting (shared with the Born worker)
     def _reserve(self, fp):
@@ -391,7 +395,23 @@ class Server:
             self.records[rid]["done"] = time.perf_counter() - t0
             self._ndone += 1
 
+    def _bury(self, job):
+        with self._lock:
+            self._graveyard.append(job)
+
+    def _flush(self):
+        """free every finished job (main thread)"""
+        with self._lock:
+            jobs, self._graveyard = self._graveyard, []
+        for j in jobs:
+            j.close()
+            self._release(footprint(j.req, self.precision))
+        return len(jobs)
+
     def _born_worker(self, inbox, t0):
+        """Born solves on a stream of this worker's own, each advanced a
+        fixed number of iterations per turn; new requests join between
+        turns, one per turn, so several workers share the arrivals."""
         p = ctypes.c_void_p()
         N.call("qsim_hip_stream_create", ctypes.byref(p))
         stream = p.value
@@ -402,31 +422,28 @@ class Server:
                     item = inbox.get(timeout=0.002) if not active else inbox.get_nowait()
                 except queue.Empty:
                     item = None
+                if item == "stop":
+                    if not active:
+                        break
+                    inbox.put("stop")
+                    item = None
                 if item is not None:
-                    if item == "stop":
-                        if not active:
-                            break
-                        inbox.put("stop")
-                    else:
-                        req = item
-                        try:
-                            active.append(BornJob(req, self.precision))
-                        except Exception as e:  # reported, never retried
-                            self.errors[req.rid] = str(e)
-                            self._finish(req.rid, t0)
-                            self._release(footprint(req, self.precision))
-                    continue
+                    try:
+                        active.append(BornJob(item, self.precision))
+                    except Exception as e:  # reported, never retried
+                        self.errors[item.rid] = str(e)
+                        self._finish(item.rid, t0)
+                        self._release(footprint(item, self.precision))
                 for job in list(active):
                     job.advance(self.quantum(job.req), stream)
                     if job.finished():
                         self.results[job.req.rid] = job.result(job.req.rid in self.keep, stream)
                         self._finish(job.req.rid, t0)
-                        job.close()
-                        self._release(footprint(job.req, self.precision))
+                        self._bury(job)
                         active.remove(job)
         finally:
             for job in active:
-                job.close()
+                self._bury(job)
             N.call("qsim_hip_stream_destroy", stream)
 
     def run(self, requests):
@@ -436,19 +453,14 @@ class Server:
         pending = collections.deque(reqs)
         waiting = collections.deque()
         lanes = {}
-        graveyard = []
         staging = _Staging()
         inbox = queue.Queue()
         t0 = time.perf_counter()
-        worker = threading.Thread(target=self._born_worker, args=(inbox, t0), daemon=True)
-        worker.start()
+        workers = [threading.Thread(target=self._born_worker, args=(inbox, t0), daemon=True)
+                   for _ in range(max(1, self.born_workers))]
+        for w in workers:
+            w.start()
         total = len(reqs)
-
-        def flush():
-            for j in graveyard:
-                j.close()
-                self._release(footprint(j.req, self.precision))
-            graveyard.clear()
 
         try:
             while self._ndone < total:
@@ -466,8 +478,7 @@ class Server:
                         self._finish(r.rid, t0)
                         continue
                     if not self._reserve(fp):
-                        if graveyard:
-                            flush()
+                        if self._flush():
                             continue
                         break
                     waiting.popleft()
@@ -494,12 +505,12 @@ class Server:
                         for j in done:
                             self.results[j.req.rid] = j.result(j.req.rid in self.keep)
                             self._finish(j.req.rid, t0)
-                            graveyard.append(j)
+                            self._bury(j)
                     if lane.inflight is None and lane.jobs:
                         lane.launch()
                     busy = busy or lane.inflight is not None
-                if not busy and graveyard:
-                    flush()
+                if not busy and self._graveyard:
+                    self._flush()
                 if not busy:
                     if pending and not waiting and inbox.empty():
                         wait = pending[0].arrival - (time.perf_counter() - t0)
@@ -509,11 +520,13 @@ class Server:
                 else:
                     time.sleep(0.00005)
         finally:
-            inbox.put("stop")
-            worker.join()
+            for _ in workers:
+                inbox.put("stop")
+            for w in workers:
+                w.join()
             for lane in lanes.values():
                 lane.close()
-            flush()
+            self._flush()
             staging.close()
         self.makespan = time.perf_counter() - t0
         return self.summary(reqs)
