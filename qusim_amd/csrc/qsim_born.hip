@@ -55,13 +55,15 @@ struct BornDev {
 __device__ __forceinline__ c128 v_eff(c128 v, const BornDev &J, int dim, size_t i, size_t j,
                                       size_t k)
 {
-    double a = dim == 3 ? (J.ab[0][i] + J.ab[1][j]) + J.ab[2][k] : J.ab[0][i] + J.ab[1][j];
+    const double a = dim == 3 ? (ld_g(J.ab[0] + i) + ld_g(J.ab[1] + j)) + ld_g(J.ab[2] + k)
+                              : ld_g(J.ab[0] + i) + ld_g(J.ab[1] + j);
     return mk(v.x, v.y + a);
 }
 
 __device__ __forceinline__ c128 psi0_at(const BornDev &J, int dim, size_t i, size_t j, size_t k)
 {
-    return dim == 3 ? (J.pw[0][i] * J.pw[1][j]) * J.pw[2][k] : J.pw[0][i] * J.pw[1][j];
+    return dim == 3 ? (ld_g(J.pw[0] + i) * ld_g(J.pw[1] + j)) * ld_g(J.pw[2] + k)
+                    : ld_g(J.pw[0] + i) * ld_g(J.pw[1] + j);
 }
 
 template <int PRE>
@@ -154,7 +156,8 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
         for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
-                v[u * RZ + r] = active ? tmp[base + P::template in_pos<RZ>(t, u, r)] : mk<T>(0, 0);
+                v[u * RZ + r] = active ? ld_g(tmp + base + P::template in_pos<RZ>(t, u, r))
+                                       : mk<T>(0, 0);
     }
     c128 vv[P::E];
     cx<T> dd[P::E];
@@ -163,13 +166,14 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
 #pragma unroll
         for (int r = 0; r < RA; ++r) {
             const int k = P::template in_pos<RA>(t, u, r);
-            vv[u * RA + r] = active ? V[base + k] : mk(0.0, 0.0);
-            dd[u * RA + r] = active ? dp[base + k] : mk<T>(0, 0);
+            vv[u * RA + r] = active ? ld_g(V + base + k) : mk(0.0, 0.0);
+            dd[u * RA + r] = active ? ld_g(dp + base + k) : mk<T>(0, 0);
         }
     if constexpr (MODE != BA_START) P::inverse(v, t, lds + l * LP::LSTR, tw);
     // per-line factors of the separable tables; the inner index varies below
-    const double ab_line = dim == 3 ? J.ab[0][i] + J.ab[1][j] : J.ab[0][i];
-    const c128 pw_line = dim == 3 ? J.pw[0][i] * J.pw[1][j] : J.pw[0][i];
+    const double ab_line =
+        dim == 3 ? ld_g(J.ab[0] + i) + ld_g(J.ab[1] + j) : ld_g(J.ab[0] + i);
+    const c128 pw_line = dim == 3 ? ld_g(J.pw[0] + i) * ld_g(J.pw[1] + j) : ld_g(J.pw[0] + i);
     const double *ab_in = J.ab[dim == 3 ? 2 : 1];
     const c128 *pw_in = J.pw[dim == 3 ? 2 : 1];
     double rsum = 0, nsum = 0;
@@ -180,7 +184,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
             const int k = P::template in_pos<RA>(t, u, r);
             const c128 vk = vv[u * RA + r];
             // v_eff: (ab0 + ab1) + ab2 in 3D, ab0 + ab1 in 2D (reference order)
-            const c128 ve = PRE == PRE_NAIVE ? vk : mk(vk.x, vk.y + (ab_line + ab_in[k]));
+            const c128 ve = PRE == PRE_NAIVE ? vk : mk(vk.x, vk.y + (ab_line + ld_g(ab_in + k)));
             c128 d = cvt<double>(dd[u * RA + r]);
             if constexpr (MODE != BA_START) {
                 const c128 tn = cvt<double>(v[u * RA + r]) * J.inv_n;
@@ -195,7 +199,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
                 d = ndr;
             }
             if constexpr (MODE != BA_END) {
-                const c128 p0 = pw_line * pw_in[k];
+                const c128 p0 = pw_line * ld_g(pw_in + k);
                 v[u * RA + r] = cvt<T>(source<PRE>(d, p0, ve, J.eps));
             }
         }
@@ -212,6 +216,128 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
     if constexpr (MODE != BA_START) {
         // block sums of the update residual and the new norm
         block_sum2<LP::THREADS>(rsum, nsum, slots);
+        if (tid == 0) {
+            J.resid[2 * blockIdx.x] = rsum;
+            J.resid[2 * blockIdx.x + 1] = nsum;
+        }
+    }
+}
+
+// y-z plane sweep (3D, FP32 iterates, n1 = n2 = 128): one workgroup per
+// (x plane, energy) holds the contiguous 128 x 128 plane.  It replaces the
+// middle-axis passes and the inner sweep: END runs IFFT_y, IFFT_z, the
+// preconditioned update and residual sums; START runs Add3, FFT_z, FFT_y.
+// With a complex64 copy of V an iteration moves 72 B per point (END 32,
+// START 24, outer 16) instead of 96.  y lines are strided in the plane: they are loaded line-fast (lane ->
+// z position), so every load and store is coalesced; the switch between
+// y lines and z lines is a transpose through LDS.
+constexpr int PLANE_N = 128;
+using PlaneLine = Pow2Line<float, PLANE_N, 16, 8, 1>;   // E = 16, 8 threads per line
+constexpr int PLANE_THREADS = PLANE_N * PlaneLine::TPL;   // 1024
+constexpr int PLANE_LS = PlaneLine::LS;                   // exchange stride (1 mod 16)
+constexpr int PLANE_TS = PLANE_N + 1;                     // transpose row stride
+constexpr size_t PLANE_LDS =
+    (size_t)(PLANE_N * PLANE_LS > PLANE_N * PLANE_TS ? PLANE_N * PLANE_LS : PLANE_N * PLANE_TS) *
+    sizeof(c64);
+
+template <int MODE, int PRE>
+__global__ void __launch_bounds__(PLANE_THREADS)
+    k_born_plane(const BornDev *__restrict__ jobs, const c64 *__restrict__ Vf, BornGeom geo,
+                 const c128 *__restrict__ tw)
+{
+    using P = PlaneLine;
+    constexpr int N = PLANE_N;
+    constexpr int RA = P::RA_, RZ = P::RLAST;
+    extern __shared__ unsigned char smem_raw[];
+    c64 *lds = reinterpret_cast<c64 *>(smem_raw);
+    __shared__ double slots[2 * (PLANE_THREADS / 64)];
+    const BornDev &J = jobs[blockIdx.y];
+    const int tid = threadIdx.x;
+    const size_t i = blockIdx.x;                  // x index of the plane
+    const size_t pbase = i * (size_t)N * N;
+    c64 *tmp = reinterpret_cast<c64 *>(J.tmp);
+    c64 *dp = reinterpret_cast<c64 *>(J.dpsi);
+    // z-line (row) layout: line j, lane t; y-line (line-fast) layout: z
+    // position kz, lane ty
+    const int j = tid / P::TPL, t = tid % P::TPL;
+    const int kz = tid % N, ty = tid / N;
+    // dpsi at the z-line positions the update visits, loaded before the
+    // inverse transforms so its latency hides behind them (V follows later:
+    // prefetching both would spill at 128 registers)
+    c64 dd[P::E];
+#pragma unroll
+    for (int r = 0; r < RA; ++r)
+        dd[r] = ld_g(dp + pbase + (size_t)j * N + P::template in_pos<RA>(t, 0, r));
+    c64 v[P::E];
+    if constexpr (MODE != BA_START) {
+        // y lines, inverse input order
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                v[u * RZ + r] = ld_g(tmp + pbase + (size_t)P::template in_pos<RZ>(ty, u, r) * N + kz);
+        P::inverse(v, ty, lds + kz * PLANE_LS, tw);
+        // transpose: y lines (outputs at in_pos<RA>) -> z lines (inputs at in_pos<RZ>)
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RA; ++r) lds[P::template in_pos<RA>(ty, 0, r) * PLANE_TS + kz] = v[r];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                v[u * RZ + r] = lds[j * PLANE_TS + P::template in_pos<RZ>(t, u, r)];
+        __syncthreads();
+        P::inverse(v, t, lds + j * PLANE_LS, tw);
+    }
+    const double ab_line = ld_g(J.ab[0] + i) + ld_g(J.ab[1] + j);
+    const c128 pw_line = ld_g(J.pw[0] + i) * ld_g(J.pw[1] + j);
+    double rsum = 0, nsum = 0;
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+        const int k = P::template in_pos<RA>(t, 0, r);
+        const c128 vk = cvt<double>(ld_g(Vf + pbase + (size_t)j * N + k));
+        const c128 ve = PRE == PRE_NAIVE ? vk : mk(vk.x, vk.y + (ab_line + ld_g(J.ab[2] + k)));
+        c128 d = cvt<double>(dd[r]);
+        if constexpr (MODE != BA_START) {
+            const c128 tn = cvt<double>(v[r]) * J.inv_n;
+            const c128 nd = update<PRE>(d, tn, ve, J.slow, J.eps);
+            const c64 nds = cvt<float>(nd);
+            store_nt(&dp[pbase + (size_t)j * N + k], nds);
+            const c128 ndr = cvt<double>(nds);
+            rsum += abs2(ndr - d);
+            nsum += abs2(ndr);
+            d = ndr;
+        }
+        if constexpr (MODE != BA_END) {
+            const c128 p0 = pw_line * ld_g(J.pw[2] + k);
+            v[r] = cvt<float>(source<PRE>(d, p0, ve, J.eps));
+        }
+    }
+    if constexpr (MODE != BA_END) {
+        if constexpr (MODE != BA_START) __syncthreads();  // LDS free again
+        P::forward(v, t, lds + j * PLANE_LS, tw);
+        // transpose: z lines (outputs at in_pos<RZ>) -> y lines (inputs at in_pos<RA>)
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                lds[j * PLANE_TS + P::template in_pos<RZ>(t, u, r)] = v[u * RZ + r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RA; ++r) v[r] = lds[P::template in_pos<RA>(ty, 0, r) * PLANE_TS + kz];
+        __syncthreads();
+        P::forward(v, ty, lds + kz * PLANE_LS, tw);
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                store_nt(&tmp[pbase + (size_t)P::template in_pos<RZ>(ty, u, r) * N + kz],
+                         v[u * RZ + r]);
+    }
+    if constexpr (MODE != BA_START) {
+        block_sum2<PLANE_THREADS>(rsum, nsum, slots);
         if (tid == 0) {
             J.resid[2 * blockIdx.x] = rsum;
             J.resid[2 * blockIdx.x + 1] = nsum;
@@ -247,7 +373,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
     for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
         for (int r = 0; r < RA; ++r)
-            v[u * RA + r] = active ? tmp[c + (size_t)P::template in_pos<RA>(t, u, r) * S]
+            v[u * RA + r] = active ? ld_g(tmp + c + (size_t)P::template in_pos<RA>(t, u, r) * S)
                                    : mk<T>(0, 0);
     P::forward(v, t, lds + l * LP::LSTR, tw);
 #pragma unroll
@@ -298,7 +424,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
     for (int u = 0; u < P::E / R0; ++u)
 #pragma unroll
         for (int r = 0; r < R0; ++r)
-            v[u * R0 + r] = active ? tmp[base + (size_t)P::template in_pos<R0>(t, u, r) * S]
+            v[u * R0 + r] = active ? ld_g(tmp + base + (size_t)P::template in_pos<R0>(t, u, r) * S)
                                    : mk<T>(0, 0);
     if constexpr (INV)
         P::inverse(v, t, lds + l * LP::LSTR, tw);
@@ -355,6 +481,10 @@ struct qsim_born_job {
     std::vector<int> iters;
     std::vector<double> resid;
     size_t inner_blocks = 0;
+    // 3D FP32 jobs with 128 x 128 y-z planes run the plane sweep; it reads a
+    // complex64 copy of V
+    bool plane = false;
+    c64 *Vf = nullptr;
     const void *tw[3] = {nullptr, nullptr, nullptr};
     // unfused path (FP64, any size)
     qsim_fft_plan *fwd = nullptr, *inv = nullptr;
@@ -454,6 +584,48 @@ int launch_inner_n(int mode, int pre, const BornDev *tab, int nact, const c128 *
     if (mode == BA_START) return launch_inner_m<T, N, BA_START>(pre, tab, nact, V, geo, tw, s);
     if (mode == BA_END) return launch_inner_m<T, N, BA_END>(pre, tab, nact, V, geo, tw, s);
     return launch_inner_m<T, N, BA_BOTH>(pre, tab, nact, V, geo, tw, s);
+}
+
+template <int MODE, int PRE>
+int launch_plane_t(const BornDev *tab, int nact, const c64 *Vf, const BornGeom &geo,
+                   const void *tw, hipStream_t s)
+{
+    auto fn = k_born_plane<MODE, PRE>;
+    QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)PLANE_LDS));
+    dim3 grid((unsigned)geo.n0, (unsigned)nact);
+    hipLaunchKernelGGL(fn, grid, dim3(PLANE_THREADS), PLANE_LDS, s, tab, Vf, geo,
+                       (const c128 *)tw);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+template <int MODE>
+int launch_plane_m(int pre, const BornDev *tab, int nact, const c64 *Vf, const BornGeom &geo,
+                   const void *tw, hipStream_t s)
+{
+    switch (pre) {
+        case PRE_IDENTITY: return launch_plane_t<MODE, PRE_IDENTITY>(tab, nact, Vf, geo, tw, s);
+        case PRE_VELLEKOOP: return launch_plane_t<MODE, PRE_VELLEKOOP>(tab, nact, Vf, geo, tw, s);
+        case PRE_HAO1: return launch_plane_t<MODE, PRE_HAO1>(tab, nact, Vf, geo, tw, s);
+        case PRE_HAO2: return launch_plane_t<MODE, PRE_HAO2>(tab, nact, Vf, geo, tw, s);
+        default: return launch_plane_t<MODE, PRE_NAIVE>(tab, nact, Vf, geo, tw, s);
+    }
+}
+
+int plane(int mode, int pre, const BornDev *tab, int nact, const c64 *Vf, const BornGeom &geo,
+          const void *tw, hipStream_t s)
+{
+    if (geo.dim != 3 || geo.n1 != (size_t)PLANE_N || geo.n2 != (size_t)PLANE_N)
+        return set_error(QSIM_ERR_UNSUPPORTED, "plane sweep needs 128 x 128 y-z planes");
+    if (mode == BA_START) return launch_plane_m<BA_START>(pre, tab, nact, Vf, geo, tw, s);
+    // the merged end + start needs more than the 128 registers a 1024-thread
+    // workgroup allows (it spilled ~200 B per lane), so it runs as two
+    // launches: 56 B per point instead of 40, still below the 80 of the
+    // inner + middle passes it replaces
+    QSIM_TRY(launch_plane_m<BA_END>(pre, tab, nact, Vf, geo, tw, s));
+    if (mode == BA_END) return QSIM_OK;
+    return launch_plane_m<BA_START>(pre, tab, nact, Vf, geo, tw, s);
 }
 
 int inner(int prec, int mode, int pre, const BornDev *tab, int nact, const c128 *V,
@@ -603,7 +775,9 @@ int create(qsim_born_job *j)
     j->started.assign(j->nE, false);
     j->iters.assign(j->nE, 0);
     j->resid.assign(j->nE, 0.0);
-    j->inner_blocks = j->fused ? inner_blocks(j->prec, g) : 0;
+    j->plane = j->fused && j->prec == 32 && dim == 3 && g.n1 == (size_t)PLANE_N &&
+               g.n2 == (size_t)PLANE_N;
+    j->inner_blocks = j->plane ? g.n0 : (j->fused ? inner_blocks(j->prec, g) : 0);
     for (int e = 0; e < j->nE; ++e) {
         BornDev &J = j->host[e];
         const double En = j->E[e];
@@ -662,6 +836,13 @@ int create(qsim_born_job *j)
         J.resid = (double *)p;
     }
     void *p;
+    if (j->plane) {
+        QSIM_TRY(balloc(j, &p, j->n * sizeof(c64)));
+        j->Vf = (c64 *)p;
+        c64 *vf = j->Vf;
+        const c128 *V = j->V;
+        QSIM_TRY(each_b(j->n, nullptr, [=] __device__(size_t idx) { vf[idx] = cvt<float>(V[idx]); }));
+    }
     QSIM_TRY(balloc(j, &p, sizeof(BornDev) * j->nE));
     j->dev = (BornDev *)p;
     QSIM_HIP(hipMemcpy(j->dev, j->host.data(), sizeof(BornDev) * j->nE, hipMemcpyHostToDevice));
@@ -693,6 +874,10 @@ int fused_pass_set(qsim_born_job *j, int mode, const BornDev *tab, int nact, hip
     // mode: BA_START or BA_BOTH starts an iteration and runs it up to the
     // point where only the inner inverse + update remains
     const void *tw_in = j->geo.dim == 3 ? j->tw[2] : j->tw[1];
+    if (j->plane) {
+        QSIM_TRY(plane(mode, j->pre, tab, nact, j->Vf, j->geo, tw_in, s));
+        return outer(j->prec, tab, nact, j->geo, j->tw[0], s);
+    }
     QSIM_TRY(inner(j->prec, mode, j->pre, tab, nact, j->V, j->geo, tw_in, s));
     if (j->geo.dim == 3) QSIM_TRY(mid(j->prec, false, tab, nact, j->geo, j->tw[1], s));
     QSIM_TRY(outer(j->prec, tab, nact, j->geo, j->tw[0], s));
@@ -770,8 +955,12 @@ int fused_iterate(qsim_born_job *j, int max_iter, double tol, hipStream_t s)
     }
     if (!act.empty()) {
         // finish the started iteration of every still-active energy
-        QSIM_TRY(inner(j->prec, BA_END, j->pre, j->active_tab, (int)act.size(), j->V, j->geo,
-                       j->geo.dim == 3 ? j->tw[2] : j->tw[1], s));
+        if (j->plane)
+            QSIM_TRY(plane(BA_END, j->pre, j->active_tab, (int)act.size(), j->Vf, j->geo,
+                           j->tw[2], s));
+        else
+            QSIM_TRY(inner(j->prec, BA_END, j->pre, j->active_tab, (int)act.size(), j->V,
+                           j->geo, j->geo.dim == 3 ? j->tw[2] : j->tw[1], s));
         std::vector<double> r;
         QSIM_TRY(read_residuals(j, act, s, r));
         for (size_t a = 0; a < act.size(); ++a) {

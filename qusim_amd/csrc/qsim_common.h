@@ -106,6 +106,31 @@ __device__ __forceinline__ void store_nt(cx<T> *p, cx<T> v)
     __builtin_nontemporal_store(v.y, (GT *)&p->y);
 }
 
+// Pointers that kernels read from descriptor structs are generic (flat) to
+// the compiler, and flat accesses count against the LDS counter too: an LDS
+// wait inside an FFT exchange would also wait for every prefetched global
+// load.  ld_g / st_g access through global-address-space pointers instead.
+template <typename T>
+__device__ __forceinline__ cx<T> ld_g(const cx<T> *p)
+{
+    using GT = __attribute__((address_space(1))) const T;
+    const GT *q = (const GT *)&p->x;
+    return mk(q[0], q[1]);
+}
+__device__ __forceinline__ double ld_g(const double *p)
+{
+    using GT = __attribute__((address_space(1))) const double;
+    return *(const GT *)p;
+}
+template <typename T>
+__device__ __forceinline__ void st_g(cx<T> *p, cx<T> v)
+{
+    using GT = __attribute__((address_space(1))) T;
+    GT *q = (GT *)&p->x;
+    q[0] = v.x;
+    q[1] = v.y;
+}
+
 __host__ __device__ __forceinline__ size_t fold_half(size_t i, size_t n)
 {
     return (i > n / 2) ? (n - i) : i;

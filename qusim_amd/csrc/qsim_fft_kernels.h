@@ -44,7 +44,9 @@ QSIM_P2CFG(double, 2048, 5, 16, 16, 8, 4, 4)
 QSIM_P2CFG4(double, 2048, 6, 8, 8, 8, 4, 1, 4)
 QSIM_P2CFG4(double, 2048, 7, 8, 8, 8, 4, 1, 2)
 QSIM_P2CFG(double, 4096, 0, 16, 16, 16, 0, 0)
-QSIM_P2CFG(double, 8192, 0, 16, 16, 32, 0, 0)  // 1D split jobs only
+// 16 elements per thread keeps these within 128 registers (32 per thread
+// spilled ~1 KiB per lane); radix 8 passes keep the padding within 160 KiB
+QSIM_P2CFG4(double, 8192, 0, 16, 8, 8, 8, 0, 0)  // 1D split jobs only
 QSIM_P2CFG(float, 16, 0, 4, 4, 1, 0, 0)
 QSIM_P2CFG(float, 32, 0, 8, 4, 1, 0, 0)
 QSIM_P2CFG(float, 64, 0, 8, 8, 1, 0, 0)
@@ -69,8 +71,8 @@ QSIM_P2CFG(float, 2048, 5, 16, 16, 8, 1, 4)
 QSIM_P2CFG4(float, 2048, 6, 8, 8, 8, 4, 1, 4)
 QSIM_P2CFG4(float, 2048, 7, 8, 8, 8, 4, 1, 2)
 QSIM_P2CFG(float, 4096, 0, 16, 16, 16, 0, 0)
-QSIM_P2CFG(float, 8192, 0, 16, 16, 32, 0, 0)
-QSIM_P2CFG(float, 16384, 0, 16, 32, 32, 0, 0)  // 1D split jobs only
+QSIM_P2CFG4(float, 8192, 0, 16, 8, 8, 8, 0, 0)
+QSIM_P2CFG(float, 16384, 0, 16, 32, 32, 0, 0)  // 1D split jobs only (32 per thread: 512 threads, 256 registers)
 #undef QSIM_P2CFG
 #undef QSIM_P2CFG4
 

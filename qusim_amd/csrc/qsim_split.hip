@@ -76,6 +76,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
     const size_t base = (active ? g : 0) * N;
     const c128 *__restrict__ ev = J.ev[iev];
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
+    c128 *psi = J.psi;
     constexpr int RA = P::RA_, RZ = P::RLAST;
     cx<T> v[P::E];
     if constexpr (MODE == SW_FIRST) {
@@ -85,7 +86,8 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
             for (int r = 0; r < RA; ++r) {
                 const int pos = P::template in_pos<RA>(t, u, r);
                 v[u * RA + r] =
-                    active ? apply(conv<T>(J.psi[base + pos]), ev[base + pos]) : mk<T>(0, 0);
+                    active ? apply(conv<T>(ld_g(psi + base + pos)), ld_g(ev + base + pos))
+                           : mk<T>(0, 0);
             }
     } else {
 #pragma unroll
@@ -93,7 +95,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
                 v[u * RZ + r] =
-                    active ? work[base + P::template in_pos<RZ>(t, u, r)] : mk<T>(0, 0);
+                    active ? ld_g(work + base + P::template in_pos<RZ>(t, u, r)) : mk<T>(0, 0);
         // the potential factors do not depend on the transform: issue their
         // loads first so they land while the inverse FFT runs
         constexpr bool PREFETCH = P::E <= 16;
@@ -103,7 +105,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
             for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
                 for (int r = 0; r < RA; ++r)
-                    evp[u * RA + r] = active ? ev[base + P::template in_pos<RA>(t, u, r)]
+                    evp[u * RA + r] = active ? ld_g(ev + base + P::template in_pos<RA>(t, u, r))
                                              : mk(0.0, 0.0);
         }
         P::inverse(v, t, lds + l * LP::LSTR, tw);
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
                 if constexpr (PREFETCH) {
                     v[u * RA + r] = apply(v[u * RA + r], evp[u * RA + r]);
                 } else {
-                    if (active) v[u * RA + r] = apply(v[u * RA + r], ev[base + pos]);
+                    if (active) v[u * RA + r] = apply(v[u * RA + r], ld_g(ev + base + pos));
                 }
             }
     }
@@ -125,7 +127,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
             for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
                 for (int r = 0; r < RA; ++r)
-                    store_nt(&J.psi[base + P::template in_pos<RA>(t, u, r)],
+                    store_nt(&psi[base + P::template in_pos<RA>(t, u, r)],
                              conv<double>(v[u * RA + r]));
         }
     } else {
@@ -159,7 +161,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
     const size_t c = active ? j : 0;
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
     const c128 *__restrict__ kx = J.kx[ik];
-    const c128 kyj = J.ky[ik][c];
+    const c128 kyj = ld_g(J.ky[ik] + c);
     constexpr int RA = P::RA_, RZ = P::RLAST;
     cx<T> v[P::E];
 #pragma unroll
@@ -167,14 +169,15 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
 #pragma unroll
         for (int r = 0; r < RA; ++r)
             v[u * RA + r] =
-                active ? work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols] : mk<T>(0, 0);
+                active ? ld_g(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
+                       : mk<T>(0, 0);
     P::forward(v, t, lds + l * LP::LSTR, tw);
 #pragma unroll
     for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
         for (int r = 0; r < RZ; ++r) {
             const int k = P::template in_pos<RZ>(t, u, r);
-            v[u * RZ + r] = apply(v[u * RZ + r], kx[k] * kyj);
+            v[u * RZ + r] = apply(v[u * RZ + r], ld_g(kx + k) * kyj);
         }
     P::inverse(v, t, lds + l * LP::LSTR, tw);
     if (active) {
@@ -207,34 +210,42 @@ __global__ void __launch_bounds__((P2Cfg<T, N>::Line::TPL))
     const int t = threadIdx.x;
     constexpr int RA = P::RA_, RZ = P::RLAST;
     cx<T> v[P::E];
+    c128 *psi = J.psi;
     const c128 *__restrict__ e0 = J.ev[prog.ev0];
 #pragma unroll
     for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
         for (int r = 0; r < RA; ++r) {
             const int pos = P::template in_pos<RA>(t, u, r);
-            v[u * RA + r] = apply(conv<T>(J.psi[pos]), e0[pos]);
+            v[u * RA + r] = apply(conv<T>(ld_g(psi + pos)), ld_g(e0 + pos));
         }
     for (size_t s = 0; s < nsteps; ++s) {
         for (int ph = 0; ph < prog.nph; ++ph) {
-            P::forward(v, t, lds, tw);
-            const c128 *__restrict__ kt = J.kx[prog.k[ph]];
+            // the twiddle loads do not change from step to step; an opaque
+            // pointer keeps the compiler from hoisting every power of every
+            // pass out of the step loop into registers (that spilled)
+            const c128 *twl = tw;
+            __asm__ volatile("" : "+s"(twl));
+            P::forward(v, t, lds, twl);
+            const c128 *kt = J.kx[prog.k[ph]];
+            __asm__ volatile("" : "+s"(kt));  // likewise: reload each step
 #pragma unroll
             for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
                 for (int r = 0; r < RZ; ++r) {
                     const int k = P::template in_pos<RZ>(t, u, r);
-                    v[u * RZ + r] = apply(v[u * RZ + r], kt[k]);
+                    v[u * RZ + r] = apply(v[u * RZ + r], ld_g(kt + k));
                 }
-            P::inverse(v, t, lds, tw);
+            P::inverse(v, t, lds, twl);
             const bool last = (s + 1 == nsteps) && (ph + 1 == prog.nph);
-            const c128 *__restrict__ ev = J.ev[last ? prog.ev_end : prog.e[ph]];
+            const c128 *ev = J.ev[last ? prog.ev_end : prog.e[ph]];
+            __asm__ volatile("" : "+s"(ev));
 #pragma unroll
             for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
                 for (int r = 0; r < RA; ++r) {
                     const int pos = P::template in_pos<RA>(t, u, r);
-                    v[u * RA + r] = apply(v[u * RA + r], ev[pos]);
+                    v[u * RA + r] = apply(v[u * RA + r], ld_g(ev + pos));
                 }
         }
     }
@@ -242,7 +253,7 @@ __global__ void __launch_bounds__((P2Cfg<T, N>::Line::TPL))
     for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
         for (int r = 0; r < RA; ++r)
-            J.psi[P::template in_pos<RA>(t, u, r)] = conv<double>(v[u * RA + r]);
+            st_g(psi + P::template in_pos<RA>(t, u, r), conv<double>(v[u * RA + r]));
 }
 
 // ---------------------------------------------------------------- helpers
