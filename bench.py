@@ -247,7 +247,10 @@ def fp32_section(steps_err, steps_rate):
             "dominant_kernel": dom, "kernels": per}
 
 
-BORN_B_PER_PT = {64: 176.0, 32: 96.0}   # per energy-iteration, 3D fused passes
+# algorithmic bytes per point-iteration of the 3D fused passes: FP64 inner 80
+# + two middle-axis 32 + outer 32; FP32 with 128^2 y-z planes: plane END 32
+# (tmp, dpsi, complex64 V read; dpsi written) + plane START 24 + outer 16
+BORN_B_PER_PT = {64: 176.0, 32: 72.0}
 
 
 def born_section(max_iter=200, tol=1e-8, checks=(0, 7, 31)):

@@ -476,6 +476,7 @@ struct qsim_born_job {
     std::vector<BornDev> host;  // per energy
     BornDev *dev = nullptr;     // device copy of `host`
     BornDev *active_tab = nullptr;
+    BornDev *tab_host = nullptr;  // pinned staging of the active table
     std::vector<void *> owned;
     std::vector<bool> started;  // tmp holds a started iteration
     std::vector<int> iters;
@@ -546,9 +547,7 @@ int launch_inner_t(const BornDev *tab, int nact, const c128 *V, const BornGeom &
 {
     using LP = P2Launch<T, N, false, BornV<N>::V>;
     auto fn = k_born_inner<T, N, MODE, PRE>;
-    if (LP::LDS_BYTES > 65536)
-        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)LP::LDS_BYTES));
+    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t nlines = geo.n0 * geo.nmid;
     dim3 grid((unsigned)((nlines + LP::L - 1) / LP::L), (unsigned)nact);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, V, geo,
@@ -591,8 +590,7 @@ int launch_plane_t(const BornDev *tab, int nact, const c64 *Vf, const BornGeom &
                    const void *tw, hipStream_t s)
 {
     auto fn = k_born_plane<MODE, PRE>;
-    QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)PLANE_LDS));
+    QSIM_TRY(lds_limit((const void *)fn, PLANE_LDS));
     dim3 grid((unsigned)geo.n0, (unsigned)nact);
     hipLaunchKernelGGL(fn, grid, dim3(PLANE_THREADS), PLANE_LDS, s, tab, Vf, geo,
                        (const c128 *)tw);
@@ -653,9 +651,7 @@ int launch_outer_t(const BornDev *tab, int nact, const BornGeom &geo, const void
 {
     using LP = P2Launch<T, N, true>;
     auto fn = k_born_outer<T, N>;
-    if (LP::LDS_BYTES > 65536)
-        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)LP::LDS_BYTES));
+    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t S = geo.n1 * geo.n2;
     dim3 grid((unsigned)((S + LP::L - 1) / LP::L), (unsigned)nact);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, geo, (const c128 *)tw);
@@ -677,9 +673,7 @@ int launch_mid_t(const BornDev *tab, int nact, const BornGeom &geo, const void *
 {
     using LP = P2Launch<T, N, true>;
     auto fn = k_born_mid<T, N, INV>;
-    if (LP::LDS_BYTES > 65536)
-        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)LP::LDS_BYTES));
+    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t nlines = geo.n0 * geo.n2;
     dim3 grid((unsigned)((nlines + LP::L - 1) / LP::L), (unsigned)nact);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, geo, (const c128 *)tw);
@@ -700,6 +694,7 @@ void destroy(qsim_born_job *j)
 {
     if (!j) return;
     for (void *p : j->owned) (void)hipFree(p);
+    if (j->tab_host) (void)hipHostFree(j->tab_host);
     if (j->fwd) qsim_hip_fft_plan_destroy(j->fwd);
     if (j->inv) qsim_hip_fft_plan_destroy(j->inv);
     delete j;
@@ -848,6 +843,8 @@ int create(qsim_born_job *j)
     QSIM_HIP(hipMemcpy(j->dev, j->host.data(), sizeof(BornDev) * j->nE, hipMemcpyHostToDevice));
     QSIM_TRY(balloc(j, &p, sizeof(BornDev) * j->nE));
     j->active_tab = (BornDev *)p;
+    QSIM_HIP(hipHostMalloc(&p, sizeof(BornDev) * j->nE, hipHostMallocDefault));
+    j->tab_host = (BornDev *)p;
     if (j->fused) {
         QSIM_TRY(twiddle_table(g.n0, j->prec, &j->tw[0]));
         QSIM_TRY(twiddle_table(g.n1, j->prec, &j->tw[1]));
@@ -913,9 +910,9 @@ int fused_iterate(qsim_born_job *j, int max_iter, double tol, hipStream_t s)
     const int check_every = tol > 0 ? 4 : 0;
     int it = 0;
     auto upload_tab = [&]() -> int {
-        std::vector<BornDev> h;
-        for (int e : act) h.push_back(j->host[e]);
-        QSIM_HIP(hipMemcpyAsync(j->active_tab, h.data(), sizeof(BornDev) * h.size(),
+        // the previous upload has completed (each one ends with a stream sync)
+        for (size_t a = 0; a < act.size(); ++a) j->tab_host[a] = j->host[act[a]];
+        QSIM_HIP(hipMemcpyAsync(j->active_tab, j->tab_host, sizeof(BornDev) * act.size(),
                                 hipMemcpyHostToDevice, s));
         QSIM_HIP(hipStreamSynchronize(s));
         return QSIM_OK;

@@ -171,6 +171,10 @@ struct Scratch {
 int scratch_acquire(size_t bytes, Scratch *out);
 void scratch_release();
 int device_cu_count();
+// Raise a kernel's dynamic-LDS limit to `bytes` (no-op up to 64 KiB), once per
+// (kernel, device) for the whole process and under a lock: the attribute is
+// process state, and request loops launch from several threads at once.
+int lds_limit(const void *kernel, size_t bytes);
 }  // namespace qsim
 
 #define QSIM_HIP(call)                                                \

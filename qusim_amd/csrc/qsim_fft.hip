@@ -59,13 +59,7 @@ static int launch_p2_inst(const void *in, void *out, size_t S, size_t nlines, co
 {
     using LP = P2Launch<T, N, LF>;
     auto fn = k_fft_p2<T, N, LF, INV>;
-    static bool attr_done = false;  // one-time, benign race (idempotent)
-    if (!attr_done && LP::LDS_BYTES > 65536) {
-        QSIM_HIP(hipFuncSetAttribute((const void *)fn,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)LP::LDS_BYTES));
-    }
-    attr_done = true;
+    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t blocks = (nlines + LP::L - 1) / LP::L;
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(LP::THREADS), LP::LDS_BYTES, s,
                        (const cx<T> *)in, (cx<T> *)out, S, nlines, (const c128 *)tw, (T)scale);
@@ -145,9 +139,7 @@ static int launch_4s_inst(const void *in, void *out, size_t S, size_t nlines, si
 {
     using LP = P2Launch<T, N, LF>;
     auto fn = k_fft_4step<T, N, LF, INV, PASS>;
-    if (LP::LDS_BYTES > 65536)
-        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)LP::LDS_BYTES));
+    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t blocks = (nlines + LP::L - 1) / LP::L;
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(LP::THREADS), LP::LDS_BYTES, s,
                        (const cx<T> *)in, (cx<T> *)out, S, nlines, other, (const c128 *)tw,
@@ -261,24 +253,13 @@ int run_axis(const FftAxis &a, int precision, bool inv, const void *in, void *ou
                          a.tw, a.n, scale, s);
     }
     size_t blocks = (a.nlines + a.L - 1) / a.L;
-    static bool attr64 = false, attr32 = false;
     if (precision == 64) {
-        if (!attr64) {
-            QSIM_HIP(hipFuncSetAttribute((const void *)k_fft_generic<double>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         160 * 1024));
-            attr64 = true;
-        }
+        QSIM_TRY(lds_limit((const void *)k_fft_generic<double>, 160 * 1024));
         hipLaunchKernelGGL(k_fft_generic<double>, dim3((unsigned)blocks), dim3(256), a.lds, s,
                            (const c128 *)in, (c128 *)out, a.S, a.nlines, a.L, a.gf,
                            (const c128 *)a.tw, (int)inv, scale);
     } else {
-        if (!attr32) {
-            QSIM_HIP(hipFuncSetAttribute((const void *)k_fft_generic<float>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         160 * 1024));
-            attr32 = true;
-        }
+        QSIM_TRY(lds_limit((const void *)k_fft_generic<float>, 160 * 1024));
         hipLaunchKernelGGL(k_fft_generic<float>, dim3((unsigned)blocks), dim3(256), a.lds, s,
                            (const c64 *)in, (c64 *)out, a.S, a.nlines, a.L, a.gf,
                            (const c128 *)a.tw, (int)inv, (float)scale);

@@ -3,6 +3,8 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <set>
+#include <utility>
 
 #include "qsim_common.h"
 
@@ -67,6 +69,22 @@ int scratch_acquire(size_t bytes, Scratch *out)
 }
 
 void scratch_release() { g_scratch_mu.unlock(); }
+
+int lds_limit(const void *kernel, size_t bytes)
+{
+    if (bytes <= 65536) return QSIM_OK;
+    static std::mutex mu;
+    static std::set<std::pair<const void *, int>> done;
+    int d = 0;
+    hipError_t e = hipGetDevice(&d);
+    if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lk(mu);
+    if (done.count({kernel, d})) return QSIM_OK;
+    e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute");
+    done.insert({kernel, d});
+    return QSIM_OK;
+}
 
 int device_cu_count()
 {

@@ -376,9 +376,7 @@ int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const voi
 {
     using LP = P2Launch<T, N, false, V>;
     auto fn = k_row_sweep<T, N, MODE, V>;
-    if (LP::LDS_BYTES > 65536)
-        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)LP::LDS_BYTES));
+    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     dim3 grid((unsigned)((nrows + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, nrows, iev,
                        (const c128 *)tw);
@@ -392,9 +390,7 @@ int launch_col(const SplitDev *jobs, int njobs, size_t ncols, int ik, const void
 {
     using LP = P2Launch<T, N, true, V>;
     auto fn = k_col_sweep<T, N, V>;
-    if (LP::LDS_BYTES > 65536)
-        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)LP::LDS_BYTES));
+    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     dim3 grid((unsigned)((ncols + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, ncols, ik,
                        (const c128 *)tw);
@@ -409,9 +405,7 @@ int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps
     using P = typename P2Cfg<T, N>::Line;
     auto fn = k_split1d<T, N>;
     const size_t lds = (size_t)P::LS * sizeof(cx<T>);
-    if (lds > 65536)
-        QSIM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds));
+    QSIM_TRY(lds_limit((const void *)fn, lds));
     hipLaunchKernelGGL(fn, dim3((unsigned)njobs), dim3(P::TPL), lds, s, jobs, prog, nsteps,
                        (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
