@@ -31,10 +31,10 @@ QSIM_P2CFG(double, 1024, 1, 16, 16, 4, 4, 4)
 QSIM_P2CFG(double, 1024, 2, 32, 32, 1, 2, 4)
 QSIM_P2CFG(double, 1024, 3, 32, 32, 1, 2, 8)
 QSIM_P2CFG(double, 1024, 4, 8, 8, 16, 1, 4)
-QSIM_P2CFG(double, 1024, 5, 16, 16, 4, 2, 2)
+QSIM_P2CFG(double, 1024, 5, 16, 16, 4, 2, 8)
 // four passes, 8 elements per thread: two waves per line
 QSIM_P2CFG4(double, 1024, 6, 8, 8, 4, 4, 1, 4)
-QSIM_P2CFG4(double, 1024, 7, 4, 4, 8, 8, 2, 2)
+QSIM_P2CFG4(double, 1024, 7, 8, 8, 4, 4, 2, 8)
 QSIM_P2CFG(double, 2048, 0, 16, 16, 8, 1, 4)
 QSIM_P2CFG(double, 2048, 1, 16, 16, 8, 2, 2)
 QSIM_P2CFG(double, 2048, 2, 32, 32, 2, 1, 4)
