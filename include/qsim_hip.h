@@ -233,8 +233,11 @@ int qsim_hip_split_steps_timed(qsim_split_job *job, size_t nsteps, double *kerne
  * than direct launches); enable = 1 turns it on. */
 int qsim_hip_split_use_graph(qsim_split_job *job, int enable);
 /* Launch variants of the 2D sweep kernels (radix set, lines per workgroup):
- * 0..7 for axis lengths 1024 and 2048 (6 and 7 run four radix passes with
- * 8 elements per thread), only 0 otherwise. */
+ * 0..15 for axis lengths 1024 and 2048 (6 and 7 run four radix passes with
+ * 8 elements per thread; 8..11 give each thread two or four lines that share
+ * their twiddles; 12..15 transform two or four line groups per workgroup in
+ * sequence, loading the next group while transforming the current one),
+ * only 0 otherwise. */
 int qsim_hip_split_set_variants(qsim_split_job *job, int row_variant, int col_variant);
 /* Autotuner: times every row variant, then every column variant, each in reps
  * alternating (row, column) sweep pairs on a copy of the state, keeps the

@@ -42,13 +42,17 @@ struct SplitDev {
 };
 
 // v * f with f an FP64 factor: FP64 multiply, or for FP32 data the factor
-// split into hi + lo FP32 parts.
+// split into hi + lo FP32 parts.  The FP32 form folds the lo products into
+// the FMAs before the last rounding (mul_ff).  Rounding v * hi first and then
+// adding v * lo lost the sub-ulp part of a near-identity factor (tiny V dt)
+// the same way at every step.  At 2048^2, dt = 0.1, that bias alone was half
+// of the 1.1e-5 psi error after 1000 steps; with mul_ff the error is 5.2e-6.
 __device__ __forceinline__ c128 apply(c128 v, c128 f) { return v * f; }
 __device__ __forceinline__ c64 apply(c64 v, c128 f)
 {
-    c64 hi = cvt<float>(f);
-    c64 lo = cvt<float>(mk(f.x - (double)hi.x, f.y - (double)hi.y));
-    return v * hi + v * lo;
+    c64 hi, lo;
+    split_ff(f, hi, lo);
+    return mul_ff(v, hi, lo);
 }
 
 template <typename TO, typename TI>
@@ -58,6 +62,12 @@ __device__ __forceinline__ cx<TO> conv(cx<TI> a)
 }
 
 // ---------------------------------------------------------------- 2D sweeps
+// Each line slot (Line::TPL threads) holds M adjacent lines at a time
+// (M = 1, 2, 4): one twiddle build serves all of them.  A MID row sweep or a
+// column sweep slot transforms SEQ such groups one after another and issues
+// the loads of group q+1 before it transforms group q, so a workgroup's HBM
+// traffic overlaps its own FFT work instead of every workgroup loading,
+// transforming and storing in step.
 template <typename T, int N, int MODE, int V>
 __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
     k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
@@ -65,84 +75,147 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
 {
     using LP = P2Launch<T, N, false, V>;
     using P = typename LP::Line;
-    constexpr int L = LP::L;
+    constexpr int L = LP::L, M = LP::M, E = P::E;
+    constexpr int SEQ = MODE == SW_MID ? LP::SEQ : 1;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const SplitDev &J = jobs[blockIdx.y];
     const int tid = threadIdx.x;
     const int l = tid / P::TPL, t = tid % P::TPL;
-    const size_t g = (size_t)blockIdx.x * L + l;
-    const bool active = g < nrows;
-    const size_t base = (active ? g : 0) * N;
+    // first line of this slot; group q, line m is g0 + q*M + m
+    const size_t g0 = ((size_t)blockIdx.x * L + l) * (SEQ * M);
     const c128 *__restrict__ ev = J.ev[iev];
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
     c128 *psi = J.psi;
+    cx<T> *ldsl = lds + l * M * LP::LSTR;
     constexpr int RA = P::RA_, RZ = P::RLAST;
-    cx<T> v[P::E];
+    cx<T> v[M * E];
     if constexpr (MODE == SW_FIRST) {
 #pragma unroll
-        for (int u = 0; u < P::E / RA; ++u)
+        for (int m = 0; m < M; ++m) {
+            const bool act = g0 + m < nrows;
+            const size_t base = (act ? g0 + m : 0) * N;
 #pragma unroll
-            for (int r = 0; r < RA; ++r) {
-                const int pos = P::template in_pos<RA>(t, u, r);
-                v[u * RA + r] =
-                    active ? apply(conv<T>(ld_g(psi + base + pos)), ld_g(ev + base + pos))
-                           : mk<T>(0, 0);
-            }
-    } else {
+            for (int u = 0; u < E / RA; ++u)
 #pragma unroll
-        for (int u = 0; u < P::E / RZ; ++u)
-#pragma unroll
-            for (int r = 0; r < RZ; ++r)
-                v[u * RZ + r] =
-                    active ? ld_g(work + base + P::template in_pos<RZ>(t, u, r)) : mk<T>(0, 0);
-        // the potential factors do not depend on the transform: issue their
-        // loads first so they land while the inverse FFT runs
-        constexpr bool PREFETCH = P::E <= 16;
-        c128 evp[PREFETCH ? P::E : 1];
-        if constexpr (PREFETCH) {
-#pragma unroll
-            for (int u = 0; u < P::E / RA; ++u)
-#pragma unroll
-                for (int r = 0; r < RA; ++r)
-                    evp[u * RA + r] = active ? ld_g(ev + base + P::template in_pos<RA>(t, u, r))
-                                             : mk(0.0, 0.0);
+                for (int r = 0; r < RA; ++r) {
+                    const size_t i = base + P::template in_pos<RA>(t, u, r);
+                    v[m * E + u * RA + r] =
+                        act ? apply(conv<T>(ld_g(psi + i)), ld_g(ev + i)) : mk<T>(0, 0);
+                }
         }
-        P::inverse(v, t, lds + l * LP::LSTR, tw);
+        P::template forward<M>(v, t, ldsl, tw, LP::LSTR);
 #pragma unroll
-        for (int u = 0; u < P::E / RA; ++u)
+        for (int m = 0; m < M; ++m)
+            if (g0 + m < nrows) {
 #pragma unroll
-            for (int r = 0; r < RA; ++r) {
-                const int pos = P::template in_pos<RA>(t, u, r);
+                for (int u = 0; u < E / RZ; ++u)
+#pragma unroll
+                    for (int r = 0; r < RZ; ++r)
+                        store_nt(&work[(g0 + m) * N + P::template in_pos<RZ>(t, u, r)],
+                                 v[m * E + u * RZ + r]);
+            }
+        return;
+    } else {
+        // the potential factors do not depend on the transform: their loads
+        // are issued with the group's data so they land while the inverse
+        // FFT runs
+        constexpr bool PREFETCH = E * M <= (SEQ > 1 ? 8 : 16);
+        c128 evp[PREFETCH ? E * M : 1];
+        auto fetch = [&](int q, cx<T> *vv, c128 *ee) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const size_t g = g0 + (size_t)q * M + m;
+                const bool act = g < nrows;
+                const size_t base = (act ? g : 0) * N;
+#pragma unroll
+                for (int u = 0; u < E / RZ; ++u)
+#pragma unroll
+                    for (int r = 0; r < RZ; ++r)
+                        vv[m * E + u * RZ + r] =
+                            act ? ld_g(work + base + P::template in_pos<RZ>(t, u, r))
+                                : mk<T>(0, 0);
                 if constexpr (PREFETCH) {
-                    v[u * RA + r] = apply(v[u * RA + r], evp[u * RA + r]);
-                } else {
-                    if (active) v[u * RA + r] = apply(v[u * RA + r], ld_g(ev + base + pos));
+#pragma unroll
+                    for (int u = 0; u < E / RA; ++u)
+#pragma unroll
+                        for (int r = 0; r < RA; ++r)
+                            ee[m * E + u * RA + r] =
+                                act ? ld_g(ev + base + P::template in_pos<RA>(t, u, r))
+                                    : mk(0.0, 0.0);
                 }
             }
-    }
-    if constexpr (MODE == SW_LAST) {
-        if (active) {
+        };
+        fetch(0, v, evp);
 #pragma unroll
-            for (int u = 0; u < P::E / RA; ++u)
+        for (int q = 0; q < SEQ; ++q) {
+            cx<T> vn[M * E];
+            c128 en[PREFETCH ? M * E : 1];
+            if constexpr (SEQ > 1)
+                if (q + 1 < SEQ) fetch(q + 1, vn, en);
+            P::template inverse<M>(v, t, ldsl, tw, LP::LSTR);
 #pragma unroll
-                for (int r = 0; r < RA; ++r)
-                    store_nt(&psi[base + P::template in_pos<RA>(t, u, r)],
-                             conv<double>(v[u * RA + r]));
-        }
-    } else {
-        P::forward(v, t, lds + l * LP::LSTR, tw);
-        if (active) {
+            for (int m = 0; m < M; ++m) {
+                const size_t g = g0 + (size_t)q * M + m;
+                const bool act = g < nrows;
+                const size_t base = (act ? g : 0) * N;
 #pragma unroll
-            for (int u = 0; u < P::E / RZ; ++u)
+                for (int u = 0; u < E / RA; ++u)
 #pragma unroll
-                for (int r = 0; r < RZ; ++r)
-                    store_nt(&work[base + P::template in_pos<RZ>(t, u, r)], v[u * RZ + r]);
+                    for (int r = 0; r < RA; ++r) {
+                        cx<T> &x = v[m * E + u * RA + r];
+                        if constexpr (PREFETCH) {
+                            x = apply(x, evp[m * E + u * RA + r]);
+                        } else {
+                            if (act)
+                                x = apply(x, ld_g(ev + base + P::template in_pos<RA>(t, u, r)));
+                        }
+                    }
+            }
+            if constexpr (MODE == SW_LAST) {
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                    if (g0 + m < nrows) {
+#pragma unroll
+                        for (int u = 0; u < E / RA; ++u)
+#pragma unroll
+                            for (int r = 0; r < RA; ++r)
+                                store_nt(&psi[(g0 + m) * N + P::template in_pos<RA>(t, u, r)],
+                                         conv<double>(v[m * E + u * RA + r]));
+                    }
+            } else {
+                P::template forward<M>(v, t, ldsl, tw, LP::LSTR);
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const size_t g = g0 + (size_t)q * M + m;
+                    if (g < nrows) {
+#pragma unroll
+                        for (int u = 0; u < E / RZ; ++u)
+#pragma unroll
+                            for (int r = 0; r < RZ; ++r)
+                                store_nt(&work[g * N + P::template in_pos<RZ>(t, u, r)],
+                                         v[m * E + u * RZ + r]);
+                    }
+                }
+            }
+            if constexpr (SEQ > 1) {
+                if (q + 1 < SEQ) {
+#pragma unroll
+                    for (int i = 0; i < M * E; ++i) v[i] = vn[i];
+                    if constexpr (PREFETCH) {
+#pragma unroll
+                        for (int i = 0; i < M * E; ++i) evp[i] = en[i];
+                    }
+                }
+            }
         }
     }
 }
 
-// Column sweep over ncols columns of an nx(=N) x ncols array.
+// Column sweep over ncols columns of an nx(=N) x ncols array.  A workgroup
+// takes SEQ groups of L*M adjacent columns; in group q, slot l holds columns
+// l, l + L, ... of the group, so every load and store instruction covers L
+// adjacent columns.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
     k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
@@ -150,42 +223,78 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
 {
     using LP = P2Launch<T, N, true, V>;
     using P = typename LP::Line;
-    constexpr int L = LP::L;
+    constexpr int L = LP::L, M = LP::M, E = P::E, SEQ = LP::SEQ;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const SplitDev &J = jobs[blockIdx.y];
     const int tid = threadIdx.x;
     const int l = tid % L, t = tid / L;
-    const size_t j = xcd_tile(blockIdx.x, gridDim.x) * L + l;
-    const bool active = j < ncols;
-    const size_t c = active ? j : 0;
+    const size_t jb = xcd_tile(blockIdx.x, gridDim.x) * (L * M * SEQ) + l;
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
     const c128 *__restrict__ kx = J.kx[ik];
-    const c128 kyj = ld_g(J.ky[ik] + c);
+    const c128 *__restrict__ ky = J.ky[ik];
+    // line m of slot l sits at LDS line m*L + l (the stride pattern of the
+    // one-line tiles holds for each m)
+    cx<T> *ldsl = lds + l * LP::LSTR;
+    constexpr int LDM = L * LP::LSTR;
     constexpr int RA = P::RA_, RZ = P::RLAST;
-    cx<T> v[P::E];
+    auto col = [&](int q, int m) { return jb + (size_t)q * (L * M) + (size_t)m * L; };
+    auto fetch = [&](int q, cx<T> *vv, c128 *kk) {
 #pragma unroll
-    for (int u = 0; u < P::E / RA; ++u)
+        for (int m = 0; m < M; ++m) {
+            const size_t j = col(q, m);
+            const bool act = j < ncols;
+            const size_t c = act ? j : 0;
+            kk[m] = ld_g(ky + c);
 #pragma unroll
-        for (int r = 0; r < RA; ++r)
-            v[u * RA + r] =
-                active ? ld_g(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
-                       : mk<T>(0, 0);
-    P::forward(v, t, lds + l * LP::LSTR, tw);
+            for (int u = 0; u < E / RA; ++u)
 #pragma unroll
-    for (int u = 0; u < P::E / RZ; ++u)
-#pragma unroll
-        for (int r = 0; r < RZ; ++r) {
-            const int k = P::template in_pos<RZ>(t, u, r);
-            v[u * RZ + r] = apply(v[u * RZ + r], ld_g(kx + k) * kyj);
+                for (int r = 0; r < RA; ++r)
+                    vv[m * E + u * RA + r] =
+                        act ? ld_g(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
+                            : mk<T>(0, 0);
         }
-    P::inverse(v, t, lds + l * LP::LSTR, tw);
-    if (active) {
+    };
+    cx<T> v[M * E];
+    c128 kyj[M];
+    fetch(0, v, kyj);
 #pragma unroll
-        for (int u = 0; u < P::E / RA; ++u)
+    for (int q = 0; q < SEQ; ++q) {
+        cx<T> vn[M * E];
+        c128 kn[M];
+        if constexpr (SEQ > 1)
+            if (q + 1 < SEQ) fetch(q + 1, vn, kn);
+        P::template forward<M>(v, t, ldsl, tw, LDM);
 #pragma unroll
-            for (int r = 0; r < RA; ++r)
-                store_nt(&work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols], v[u * RA + r]);
+        for (int u = 0; u < E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) {
+                const c128 kk = ld_g(kx + P::template in_pos<RZ>(t, u, r));
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                    v[m * E + u * RZ + r] = apply(v[m * E + u * RZ + r], kk * kyj[m]);
+            }
+        P::template inverse<M>(v, t, ldsl, tw, LDM);
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const size_t j = col(q, m);
+            if (j < ncols) {
+#pragma unroll
+                for (int u = 0; u < E / RA; ++u)
+#pragma unroll
+                    for (int r = 0; r < RA; ++r)
+                        store_nt(&work[j + (size_t)P::template in_pos<RA>(t, u, r) * ncols],
+                                 v[m * E + u * RA + r]);
+            }
+        }
+        if constexpr (SEQ > 1) {
+            if (q + 1 < SEQ) {
+#pragma unroll
+                for (int i = 0; i < M * E; ++i) v[i] = vn[i];
+#pragma unroll
+                for (int m = 0; m < M; ++m) kyj[m] = kn[m];
+            }
+        }
     }
 }
 
@@ -377,7 +486,9 @@ int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const voi
     using LP = P2Launch<T, N, false, V>;
     auto fn = k_row_sweep<T, N, MODE, V>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
-    dim3 grid((unsigned)((nrows + LP::L - 1) / LP::L), (unsigned)njobs);
+    // the FIRST and LAST sweeps transform one group per slot
+    constexpr size_t per_wg = (size_t)LP::LINES * (MODE == SW_MID ? LP::SEQ : 1);
+    dim3 grid((unsigned)((nrows + per_wg - 1) / per_wg), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, nrows, iev,
                        (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
@@ -391,7 +502,8 @@ int launch_col(const SplitDev *jobs, int njobs, size_t ncols, int ik, const void
     using LP = P2Launch<T, N, true, V>;
     auto fn = k_col_sweep<T, N, V>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
-    dim3 grid((unsigned)((ncols + LP::L - 1) / LP::L), (unsigned)njobs);
+    constexpr size_t per_wg = (size_t)LP::LINES * LP::SEQ;
+    dim3 grid((unsigned)((ncols + per_wg - 1) / per_wg), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, ncols, ik,
                        (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
@@ -432,7 +544,7 @@ int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps
 
 // Launch variants: lengths 1024 and 2048 have the candidates 0..kVariants-1
 // (P2Cfg), every other length only variant 0.
-constexpr int kVariants = 8;
+constexpr int kVariants = 16;
 
 bool has_variants(size_t n) { return n == 1024 || n == 2048; }
 
@@ -449,6 +561,14 @@ int row_v(int var, const SplitDev *jobs, int njobs, size_t nrows, int iev, const
             case 5: return launch_row<T, NN, MODE, 5>(jobs, njobs, nrows, iev, tw, s);
             case 6: return launch_row<T, NN, MODE, 6>(jobs, njobs, nrows, iev, tw, s);
             case 7: return launch_row<T, NN, MODE, 7>(jobs, njobs, nrows, iev, tw, s);
+            case 8: return launch_row<T, NN, MODE, 8>(jobs, njobs, nrows, iev, tw, s);
+            case 9: return launch_row<T, NN, MODE, 9>(jobs, njobs, nrows, iev, tw, s);
+            case 10: return launch_row<T, NN, MODE, 10>(jobs, njobs, nrows, iev, tw, s);
+            case 11: return launch_row<T, NN, MODE, 11>(jobs, njobs, nrows, iev, tw, s);
+            case 12: return launch_row<T, NN, MODE, 12>(jobs, njobs, nrows, iev, tw, s);
+            case 13: return launch_row<T, NN, MODE, 13>(jobs, njobs, nrows, iev, tw, s);
+            case 14: return launch_row<T, NN, MODE, 14>(jobs, njobs, nrows, iev, tw, s);
+            case 15: return launch_row<T, NN, MODE, 15>(jobs, njobs, nrows, iev, tw, s);
             default: break;
         }
     }
@@ -468,6 +588,14 @@ int col_v(int var, const SplitDev *jobs, int njobs, size_t ncols, int ik, const 
             case 5: return launch_col<T, NN, 5>(jobs, njobs, ncols, ik, tw, s);
             case 6: return launch_col<T, NN, 6>(jobs, njobs, ncols, ik, tw, s);
             case 7: return launch_col<T, NN, 7>(jobs, njobs, ncols, ik, tw, s);
+            case 8: return launch_col<T, NN, 8>(jobs, njobs, ncols, ik, tw, s);
+            case 9: return launch_col<T, NN, 9>(jobs, njobs, ncols, ik, tw, s);
+            case 10: return launch_col<T, NN, 10>(jobs, njobs, ncols, ik, tw, s);
+            case 11: return launch_col<T, NN, 11>(jobs, njobs, ncols, ik, tw, s);
+            case 12: return launch_col<T, NN, 12>(jobs, njobs, ncols, ik, tw, s);
+            case 13: return launch_col<T, NN, 13>(jobs, njobs, ncols, ik, tw, s);
+            case 14: return launch_col<T, NN, 14>(jobs, njobs, ncols, ik, tw, s);
+            case 15: return launch_col<T, NN, 15>(jobs, njobs, ncols, ik, tw, s);
             default: break;
         }
     }

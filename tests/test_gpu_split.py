@@ -183,13 +183,13 @@ def test_1d_fp32_long_lines(n, oracle):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", range(16))
 @pytest.mark.parametrize("shape", [(1024, 64), (64, 1024), (2048, 64), (64, 2048)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_launch_variants_match_oracle(shape, variant, oracle):
     """every row / column launch variant of the 1024 and 2048 sweeps (radix
-    plans of three and four passes) against the oracle; FP32-mixed against
-    the FP64 device result"""
+    plans of three and four passes, one to four lines per thread) against the
+    oracle (pipelined forms included); FP32-mixed against the FP64 device result"""
     from qusim_amd._native import call
     psi, V, L0, L1 = gauss2d(*shape)
     outs = {}
