@@ -207,44 +207,105 @@ def pmc_summary(fetch_dir, write_dir):
     print(json.dumps(out, indent=1))
 
 
-def fp32_section(steps_err, steps_rate):
-    """FP32-mixed 2048^2 (dt=0.1): rate, and error vs the complex128 path of
-    this device after `steps_err` steps (the oracle's parity with the FP64
-    device path is established by the test suite)."""
+def batched_rate(prob, precision, njobs, steps=40, reps=10):
+    """grid-pt-steps/s of njobs same-shape jobs advanced together (one launch
+    per sweep for all of them), variants from the batched autotuner"""
     from qusim_amd import _native as Nn
-    from qusim_amd.workloads import c3_problem
-    prob = c3_problem()
+    jobs = [make_job(Nn, prob, precision) for _ in range(njobs)]
+    ptrs = [j[0] for j in jobs]
+    arr = (ctypes.c_void_p * njobs)(*ptrs)
+    vr, vc = ctypes.c_int(), ctypes.c_int()
+    ru, cu = ctypes.c_double(), ctypes.c_double()
+    Nn.call("qsim_hip_split_autotune_batched", arr, njobs, reps, ctypes.byref(vr),
+            ctypes.byref(vc), ctypes.byref(ru), ctypes.byref(cu), None)
+    b = ctypes.c_void_p()
+    Nn.call("qsim_hip_split_batch_create", ctypes.byref(b), njobs)
+    try:
+        Nn.call("qsim_hip_split_batch_steps", b, arr, njobs, 2, None)
+        Nn.synchronize()
+        e0, e1 = Nn.Event(), Nn.Event()
+        e0.record()
+        Nn.call("qsim_hip_split_batch_steps", b, arr, njobs, steps, None)
+        e1.record()
+        Nn.synchronize()
+        secs = e0.elapsed_ms(e1) / 1e3
+    finally:
+        Nn.call("qsim_hip_split_batch_destroy", b)
+        for p_ in ptrs:
+            Nn.call("qsim_hip_split_destroy", p_)
+    n = prob["n"]
+    return {"jobs": njobs, "value": njobs * steps * n * n / secs,
+            "us_per_step_per_job": secs / steps / njobs * 1e6,
+            "launch_variants": [vr.value, vc.value]}
+
+
+def fp32_error(prob, steps):
+    """psi relative L2 error and norm drift of the FP32-mixed path against the
+    complex128 path of this device after `steps` steps from the same state
+    (the complex128 path's parity with the oracle is the test suite's)"""
+    from qusim_amd import _native as Nn
     n = prob["n"]
     j64, keep64 = make_job(Nn, prob, 64)
     j32, keep32 = make_job(Nn, prob, 32)
+    Nn.call("qsim_hip_split_steps", j64, steps, None)
+    Nn.call("qsim_hip_split_steps", j32, steps, None)
+    a = keep64[0].download()
+    b = keep32[0].download()
+    for j in (j64, j32):
+        Nn.call("qsim_hip_split_destroy", j)
+    dx = (prob["x1"] - prob["x0"]) / n
+    n64 = float(np.sum(np.abs(a) ** 2) * dx * dx)
+    n32 = float(np.sum(np.abs(b) ** 2) * dx * dx)
+    return {"psi_rel_l2_vs_fp64": float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+            "steps": steps, "norm_drift_vs_fp64": (n32 - n64) / n64}
+
+
+def fp32_section(steps_err, steps_rate, fp64_plain_1024):
+    """FP32-mixed form (complex64 psi and FFTs, FP64-built factors applied as
+    float-float pairs): the 2048^2 dt = 0.1 grid alone and batched, the
+    1024^2 grid of the FP64 headline batched, and the psi error against the
+    complex128 path after `steps_err` steps on both grids."""
+    from qusim_amd import _native as Nn
+    from qusim_amd.workloads import c2_problem, c3_problem
+    p3 = c3_problem()
+    n = p3["n"]
+    j64, _k64 = make_job(Nn, p3, 64)
+    j32, _k32 = make_job(Nn, p3, 32)
     autotune(j64)
-    autotune(j32)
-    wall, gpu = time_steps(j32, steps_rate, 2)
-    rate32 = steps_rate * n * n / gpu
+    tuned32 = autotune(j32)
+    w32, g32 = time_steps(j32, steps_rate, 2)
+    rate32 = steps_rate * n * n / g32
     w64, g64 = time_steps(j64, steps_rate, 2)
     rate64 = steps_rate * n * n / g64
     ms, cnt = kernel_times(j32, 4)
     dom, per = roofline(ms, cnt, n * n, 32)
-    # error after steps_err steps from the same initial state
-    Nn.call("qsim_hip_memcpy_h2d", keep64[0].ptr, np.ascontiguousarray(prob["psi"]).ctypes.data,
-            keep64[0].nbytes)
-    Nn.call("qsim_hip_memcpy_h2d", keep32[0].ptr, np.ascontiguousarray(prob["psi"]).ctypes.data,
-            keep32[0].nbytes)
-    Nn.call("qsim_hip_split_steps", j64, steps_err, None)
-    Nn.call("qsim_hip_split_steps", j32, steps_err, None)
-    a = keep64[0].download()
-    b = keep32[0].download()
-    err = float(np.linalg.norm(b - a) / np.linalg.norm(a))
-    dx = (prob["x1"] - prob["x0"]) / n
-    n64 = float(np.sum(np.abs(a) ** 2) * dx * dx)
-    n32 = float(np.sum(np.abs(b) ** 2) * dx * dx)
     for j in (j64, j32):
         Nn.call("qsim_hip_split_destroy", j)
-    return {"workload": "2D O2 2048x2048, dt=0.1, FP32 data + FP64 factors",
-            "value": rate32, "unit": "grid-pt-steps/s", "fp64_value_same_grid": rate64,
-            "speedup_vs_fp64": rate32 / rate64, "psi_rel_l2_vs_fp64": err,
-            "err_steps": steps_err, "norm_drift_vs_fp64": (n32 - n64) / n64,
-            "dominant_kernel": dom, "kernels": per}
+    b3 = batched_rate(p3, 32, 4)
+    p2 = c2_problem()
+    b2 = batched_rate(p2, 32, 16)
+    b2_64 = batched_rate(p2, 64, 4)
+    e3 = fp32_error(p3, steps_err)
+    e2 = fp32_error(p2, steps_err)
+    return {
+        "workload": "FP32-mixed split-operator O2: complex64 psi and FFTs, factors built in "
+                    "FP64 and applied as float-float pairs",
+        "value": b2["value"], "unit": "grid-pt-steps/s",
+        "speedup_vs_fp64": b2["value"] / fp64_plain_1024,
+        "speedup_basis": "16 batched 1024^2 FP32-mixed jobs vs the plain FP64 1024^2 job",
+        "grid_1024": {"workload": "1024x1024, dt=0.2 (the FP64 headline grid)",
+                      "fp32_batched": b2, "fp64_plain": fp64_plain_1024,
+                      "fp64_batched": b2_64,
+                      "speedup_vs_fp64_plain": b2["value"] / fp64_plain_1024,
+                      "speedup_vs_fp64_batched": b2["value"] / b2_64["value"],
+                      "error": e2},
+        "grid_2048": {"workload": "2048x2048, dt=0.1",
+                      "fp32_single": rate32, "fp32_single_variants": tuned32,
+                      "fp32_batched": b3, "fp64_single": rate64,
+                      "speedup_single_vs_fp64": rate32 / rate64,
+                      "speedup_batched_vs_fp64": b3["value"] / rate64,
+                      "error": e3, "dominant_kernel": dom, "kernels": per},
+    }
 
 
 # algorithmic bytes per point-iteration of the 3D fused passes: FP64 inner 80
@@ -412,8 +473,15 @@ def main():
     }
     Nn.call("qsim_hip_split_destroy", job)
     if not args.no_fp32:
-        log("FP32-mixed 2048^2 section")
-        out["fp32_mixed"] = fp32_section(args.fp32_err_steps, 50)
+        log("FP32-mixed section (2048^2 alone and batched, 1024^2 batched)")
+        f = fp32_section(args.fp32_err_steps, 50, value)
+        out["fp32_mixed"] = f
+        log("FP32-mixed: 1024^2 x16 batched %.3e (%.2fx FP64 1024^2), 2048^2 %.3e alone, "
+            "%.3e x4 batched; psi error after %d steps %.2e (2048^2), %.2e (1024^2)"
+            % (f["value"], f["speedup_vs_fp64"], f["grid_2048"]["fp32_single"],
+               f["grid_2048"]["fp32_batched"]["value"], args.fp32_err_steps,
+               f["grid_2048"]["error"]["psi_rel_l2_vs_fp64"],
+               f["grid_1024"]["error"]["psi_rel_l2_vs_fp64"]))
     if not args.no_born:
         log("Born 128^3 x 32 energies section")
         out["born_3d"] = born_section()

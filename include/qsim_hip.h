@@ -233,11 +233,8 @@ int qsim_hip_split_steps_timed(qsim_split_job *job, size_t nsteps, double *kerne
  * than direct launches); enable = 1 turns it on. */
 int qsim_hip_split_use_graph(qsim_split_job *job, int enable);
 /* Launch variants of the 2D sweep kernels (radix set, lines per workgroup):
- * 0..15 for axis lengths 1024 and 2048 (6 and 7 run four radix passes with
- * 8 elements per thread; 8..11 give each thread two or four lines that share
- * their twiddles; 12..15 transform two or four line groups per workgroup in
- * sequence, loading the next group while transforming the current one),
- * only 0 otherwise. */
+ * 0..7 for axis lengths 1024 and 2048 (6 and 7 run four radix passes with
+ * 8 elements per thread), only 0 otherwise. */
 int qsim_hip_split_set_variants(qsim_split_job *job, int row_variant, int col_variant);
 /* Autotuner: times every row variant, then every column variant, each in reps
  * alternating (row, column) sweep pairs on a copy of the state, keeps the
@@ -245,6 +242,14 @@ int qsim_hip_split_set_variants(qsim_split_job *job, int row_variant, int col_va
  * pair at the end of the row and of the column stage.  Synchronous. */
 int qsim_hip_split_autotune(qsim_split_job *job, int reps, int *row_variant, int *col_variant,
                             double *row_us, double *col_us, qsim_stream_t s);
+/* The same search with all njobs same-shape jobs in every launch, as
+ * qsim_hip_split_steps_batched and qsim_hip_split_batch_steps run them (more
+ * workgroups per launch can favour other variants); sets the chosen pair on
+ * every job.  Synchronous.  No counterpart in the reference (its solvers run
+ * one grid at a time). */
+int qsim_hip_split_autotune_batched(qsim_split_job *const *jobs, int njobs, int reps,
+                                    int *row_variant, int *col_variant, double *row_us,
+                                    double *col_us, qsim_stream_t s);
 /* Re-read psi after the caller changed it (e.g. renormalised it). */
 int qsim_hip_split_load_psi(qsim_split_job *job, qsim_stream_t s);
 /* sum |psi|^2 over the job's working state (compensated FP64), synchronous */

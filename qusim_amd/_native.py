@@ -144,6 +144,8 @@ _SIGS = {
     "qsim_hip_born_xsection": [_VP, _I, _D, _D, _D, ctypes.POINTER(_D), ctypes.POINTER(c128),
                                _VP],
     "qsim_hip_born_destroy": [_VP],
+    "qsim_hip_split_autotune_batched": [_VP, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                        ctypes.POINTER(_D), ctypes.POINTER(_D), _VP],
     "qsim_hip_split_batch_create": [_VP, _I],
     "qsim_hip_split_batch_steps": [_VP, _VP, _I, _SZ, _VP],
     "qsim_hip_split_batch_destroy": [_VP],

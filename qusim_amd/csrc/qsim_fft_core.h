@@ -175,9 +175,8 @@ struct Pow2Line {
     // twiddle + butterfly for a pass of radix R after NS points combined:
     // one FP64 table load per butterfly, the powers w^r by FP64
     // multiplication (error ~r ulp, far below the 1e-12 bar); FP32 data
-    // takes each power as a float-float pair.  M lines per thread (line m in
-    // v[m*E ..]) share every twiddle: it is built once and applied M times.
-    template <int R, int NS, bool INV, int M = 1>
+    // takes each power as a float-float pair.
+    template <int R, int NS, bool INV>
     static __device__ __forceinline__ void pass(cx<T> *v, int t, const c128 *__restrict__ tw)
     {
 #pragma unroll
@@ -191,11 +190,7 @@ struct Pow2Line {
                     cx<T> w = w1;
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
-#pragma unroll
-                        for (int m = 0; m < M; ++m) {
-                            cx<T> &x = v[m * E + u * R + r];
-                            x = INV ? mulconj(x, w) : x * w;
-                        }
+                        v[u * R + r] = INV ? mulconj(v[u * R + r], w) : v[u * R + r] * w;
                         if (r + 1 < R) w = w * w1;
                     }
                 } else {
@@ -208,65 +203,53 @@ struct Pow2Line {
                     for (int r = 1; r < R; ++r) {
                         c64 wh, wl;
                         split_ff(w, wh, wl);
-#pragma unroll
-                        for (int m = 0; m < M; ++m) {
-                            c64 &x = v[m * E + u * R + r];
-                            x = INV ? mulconj_ff(x, wh, wl) : mul_ff(x, wh, wl);
-                        }
+                        v[u * R + r] = INV ? mulconj_ff(v[u * R + r], wh, wl)
+                                           : mul_ff(v[u * R + r], wh, wl);
                         if (r + 1 < R) w = w * w1;
                     }
                 }
             }
-#pragma unroll
-            for (int m = 0; m < M; ++m) dft_reg<T, R, INV>(v + m * E + u * R);
+            dft_reg<T, R, INV>(v + u * R);
         }
     }
 
     // LDS exchange from the outputs of a pass (radix R, after NS) to the
-    // inputs of a pass of radix R2; line m of the thread uses lds + m*lstr.
-    template <int R, int NS, int R2, int M = 1>
-    static __device__ __forceinline__ void exchange(cx<T> *v, int t, cx<T> *lds, int lstr)
+    // inputs of a pass of radix R2.
+    template <int R, int NS, int R2>
+    static __device__ __forceinline__ void exchange(cx<T> *v, int t, cx<T> *lds)
     {
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < M; ++m)
+        for (int u = 0; u < E / R; ++u) {
+            const int j = t + u * TPL;
+            const int base = (j / NS) * NS * R + (j & (NS - 1));
 #pragma unroll
-            for (int u = 0; u < E / R; ++u) {
-                const int j = t + u * TPL;
-                const int base = (j / NS) * NS * R + (j & (NS - 1));
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-                    lds[m * lstr + pad<R>(base + r * NS)] = v[m * E + u * R + r];
-            }
+            for (int r = 0; r < R; ++r) lds[pad<R>(base + r * NS)] = v[u * R + r];
+        }
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < M; ++m)
+        for (int u = 0; u < E / R2; ++u)
 #pragma unroll
-            for (int u = 0; u < E / R2; ++u)
-#pragma unroll
-                for (int r = 0; r < R2; ++r)
-                    v[m * E + u * R2 + r] = lds[m * lstr + pad<R>(in_pos<R2>(t, u, r))];
+            for (int r = 0; r < R2; ++r) v[u * R2 + r] = lds[pad<R>(in_pos<R2>(t, u, r))];
     }
 
     // Forward transform.  In: inputs of radix-RA pass (in_pos<RA>).  Out: X at
     // in_pos<RLAST>(t, u, r) with RLAST = RD, RC, RB or RA (NP = 4, 3, 2, 1).
-    // M lines per thread: v holds M*E elements, line m's LDS at lds + m*lstr.
-    template <int M = 1>
     static __device__ __forceinline__ void forward(cx<T> *v, int t, cx<T> *lds,
-                                                   const c128 *__restrict__ tw, int lstr = 0)
+                                                   const c128 *__restrict__ tw)
     {
-        pass<RA, 1, false, M>(v, t, tw);
+        pass<RA, 1, false>(v, t, tw);
         if constexpr (NP >= 2) {
-            exchange<RA, 1, RB, M>(v, t, lds, lstr);
-            pass<RB, RA, false, M>(v, t, tw);
+            exchange<RA, 1, RB>(v, t, lds);
+            pass<RB, RA, false>(v, t, tw);
         }
         if constexpr (NP >= 3) {
-            exchange<RB, RA, RC, M>(v, t, lds, lstr);
-            pass<RC, RA * RB, false, M>(v, t, tw);
+            exchange<RB, RA, RC>(v, t, lds);
+            pass<RC, RA * RB, false>(v, t, tw);
         }
         if constexpr (NP == 4) {
-            exchange<RC, RA * RB, RD, M>(v, t, lds, lstr);
-            pass<RD, RA * RB * RC, false, M>(v, t, tw);
+            exchange<RC, RA * RB, RD>(v, t, lds);
+            pass<RD, RA * RB * RC, false>(v, t, tw);
         }
     }
 
@@ -274,30 +257,29 @@ struct Pow2Line {
 
     // Inverse transform, radices RLAST, ..., RA.  In: in_pos<RLAST>.  Out: x at
     // in_pos<RA>(t, u, r).
-    template <int M = 1>
     static __device__ __forceinline__ void inverse(cx<T> *v, int t, cx<T> *lds,
-                                                   const c128 *__restrict__ tw, int lstr = 0)
+                                                   const c128 *__restrict__ tw)
     {
         if constexpr (NP == 1) {
-            pass<RA, 1, true, M>(v, t, tw);
+            pass<RA, 1, true>(v, t, tw);
         } else if constexpr (NP == 2) {
-            pass<RB, 1, true, M>(v, t, tw);
-            exchange<RB, 1, RA, M>(v, t, lds, lstr);
-            pass<RA, RB, true, M>(v, t, tw);
+            pass<RB, 1, true>(v, t, tw);
+            exchange<RB, 1, RA>(v, t, lds);
+            pass<RA, RB, true>(v, t, tw);
         } else if constexpr (NP == 3) {
-            pass<RC, 1, true, M>(v, t, tw);
-            exchange<RC, 1, RB, M>(v, t, lds, lstr);
-            pass<RB, RC, true, M>(v, t, tw);
-            exchange<RB, RC, RA, M>(v, t, lds, lstr);
-            pass<RA, RC * RB, true, M>(v, t, tw);
+            pass<RC, 1, true>(v, t, tw);
+            exchange<RC, 1, RB>(v, t, lds);
+            pass<RB, RC, true>(v, t, tw);
+            exchange<RB, RC, RA>(v, t, lds);
+            pass<RA, RC * RB, true>(v, t, tw);
         } else {
-            pass<RD, 1, true, M>(v, t, tw);
-            exchange<RD, 1, RC, M>(v, t, lds, lstr);
-            pass<RC, RD, true, M>(v, t, tw);
-            exchange<RC, RD, RB, M>(v, t, lds, lstr);
-            pass<RB, RD * RC, true, M>(v, t, tw);
-            exchange<RB, RD * RC, RA, M>(v, t, lds, lstr);
-            pass<RA, RD * RC * RB, true, M>(v, t, tw);
+            pass<RD, 1, true>(v, t, tw);
+            exchange<RD, 1, RC>(v, t, lds);
+            pass<RC, RD, true>(v, t, tw);
+            exchange<RC, RD, RB>(v, t, lds);
+            pass<RB, RD * RC, true>(v, t, tw);
+            exchange<RB, RD * RC, RA>(v, t, lds);
+            pass<RA, RD * RC * RB, true>(v, t, tw);
         }
     }
 };

@@ -12,21 +12,12 @@ namespace qsim {
 // (contiguous) and column (strided) kernels; 0 = derive from LDS.
 template <typename T, int N, int V = 0>
 struct P2Cfg;
-// ML: lines per thread (they share every twiddle); ROW_L / COL_L then count
-// groups of ML lines.
-// SQ: groups of ML lines a slot transforms one after another (the split
-// sweeps load group q+1 while they transform group q).
-#define QSIM_P2CFGS(TYPE, N_, V_, A, B, C_, D_, RL, CL, ML_, SQ_) \
-    template <>                                                   \
-    struct P2Cfg<TYPE, N_, V_> {                                  \
-        using Line = Pow2Line<TYPE, N_, A, B, C_, D_>;            \
-        static constexpr int ROW_L = RL, COL_L = CL;              \
-        static constexpr int ML = ML_, SQ = SQ_;                  \
-    };
-#define QSIM_P2CFGM(TYPE, N_, V_, A, B, C_, D_, RL, CL, ML_) \
-    QSIM_P2CFGS(TYPE, N_, V_, A, B, C_, D_, RL, CL, ML_, 1)
 #define QSIM_P2CFG4(TYPE, N_, V_, A, B, C_, D_, RL, CL) \
-    QSIM_P2CFGM(TYPE, N_, V_, A, B, C_, D_, RL, CL, 1)
+    template <>                                          \
+    struct P2Cfg<TYPE, N_, V_> {                         \
+        using Line = Pow2Line<TYPE, N_, A, B, C_, D_>;   \
+        static constexpr int ROW_L = RL, COL_L = CL;     \
+    };
 #define QSIM_P2CFG(TYPE, N_, V_, A, B, C_, RL, CL) QSIM_P2CFG4(TYPE, N_, V_, A, B, C_, 1, RL, CL)
 QSIM_P2CFG(double, 16, 0, 4, 4, 1, 0, 0)
 QSIM_P2CFG(double, 32, 0, 8, 4, 1, 0, 0)
@@ -44,16 +35,6 @@ QSIM_P2CFG(double, 1024, 5, 16, 16, 4, 2, 8)
 // four passes, 8 elements per thread: two waves per line
 QSIM_P2CFG4(double, 1024, 6, 8, 8, 4, 4, 1, 4)
 QSIM_P2CFG4(double, 1024, 7, 8, 8, 4, 4, 2, 8)
-// two lines per thread
-QSIM_P2CFGM(double, 1024, 8, 8, 8, 4, 4, 2, 2, 2)
-QSIM_P2CFGM(double, 1024, 9, 16, 16, 4, 1, 2, 2, 2)
-QSIM_P2CFGM(double, 1024, 10, 8, 8, 4, 4, 1, 1, 2)
-QSIM_P2CFGM(double, 1024, 11, 8, 8, 16, 1, 2, 2, 2)
-// groups transformed in sequence, the next group's loads in flight
-QSIM_P2CFGS(double, 1024, 12, 8, 8, 4, 4, 1, 2, 1, 2)
-QSIM_P2CFGS(double, 1024, 13, 8, 8, 4, 4, 2, 2, 1, 2)
-QSIM_P2CFGS(double, 1024, 14, 8, 8, 16, 1, 1, 2, 1, 2)
-QSIM_P2CFGS(double, 1024, 15, 16, 16, 4, 1, 2, 2, 1, 2)
 QSIM_P2CFG(double, 2048, 0, 16, 16, 8, 1, 4)
 QSIM_P2CFG(double, 2048, 1, 16, 16, 8, 2, 2)
 QSIM_P2CFG(double, 2048, 2, 32, 32, 2, 1, 4)
@@ -62,14 +43,6 @@ QSIM_P2CFG(double, 2048, 4, 16, 16, 8, 1, 8)
 QSIM_P2CFG(double, 2048, 5, 16, 16, 8, 4, 4)
 QSIM_P2CFG4(double, 2048, 6, 8, 8, 8, 4, 1, 4)
 QSIM_P2CFG4(double, 2048, 7, 8, 8, 8, 4, 1, 2)
-QSIM_P2CFGM(double, 2048, 8, 8, 8, 8, 4, 1, 1, 2)
-QSIM_P2CFGM(double, 2048, 9, 16, 16, 8, 1, 1, 1, 2)
-QSIM_P2CFGM(double, 2048, 10, 16, 16, 8, 1, 1, 2, 2)
-QSIM_P2CFGM(double, 2048, 11, 8, 16, 16, 1, 1, 2, 2)
-QSIM_P2CFGS(double, 2048, 12, 8, 8, 8, 4, 1, 2, 1, 2)
-QSIM_P2CFGS(double, 2048, 13, 8, 8, 8, 4, 1, 2, 1, 4)
-QSIM_P2CFGS(double, 2048, 14, 16, 16, 8, 1, 1, 2, 1, 2)
-QSIM_P2CFGS(double, 2048, 15, 16, 16, 8, 1, 2, 4, 1, 2)
 QSIM_P2CFG(double, 4096, 0, 16, 16, 16, 0, 0)
 // 16 elements per thread keeps these within 128 registers (32 per thread
 // spilled ~1 KiB per lane); radix 8 passes keep the padding within 160 KiB
@@ -89,15 +62,6 @@ QSIM_P2CFG(float, 1024, 4, 32, 32, 1, 2, 4)
 QSIM_P2CFG(float, 1024, 5, 16, 16, 4, 4, 4)
 QSIM_P2CFG4(float, 1024, 6, 8, 8, 4, 4, 2, 8)
 QSIM_P2CFG4(float, 1024, 7, 8, 8, 4, 4, 1, 4)
-// two or four lines per thread: one twiddle build per two or four lines
-QSIM_P2CFGM(float, 1024, 8, 8, 8, 4, 4, 2, 4, 2)
-QSIM_P2CFGM(float, 1024, 9, 16, 16, 4, 1, 2, 4, 2)
-QSIM_P2CFGM(float, 1024, 10, 8, 8, 4, 4, 1, 2, 4)
-QSIM_P2CFGM(float, 1024, 11, 8, 8, 16, 1, 2, 4, 2)
-QSIM_P2CFGS(float, 1024, 12, 8, 8, 4, 4, 1, 2, 1, 2)
-QSIM_P2CFGS(float, 1024, 13, 8, 8, 4, 4, 2, 2, 1, 2)
-QSIM_P2CFGS(float, 1024, 14, 16, 16, 4, 1, 2, 2, 1, 2)
-QSIM_P2CFGS(float, 1024, 15, 8, 8, 4, 4, 1, 2, 2, 2)
 QSIM_P2CFG(float, 2048, 0, 16, 16, 8, 1, 8)
 QSIM_P2CFG(float, 2048, 1, 32, 64, 1, 1, 8)
 QSIM_P2CFG(float, 2048, 2, 64, 32, 1, 1, 8)
@@ -106,47 +70,31 @@ QSIM_P2CFG(float, 2048, 4, 32, 32, 2, 1, 8)
 QSIM_P2CFG(float, 2048, 5, 16, 16, 8, 1, 4)
 QSIM_P2CFG4(float, 2048, 6, 8, 8, 8, 4, 1, 4)
 QSIM_P2CFG4(float, 2048, 7, 8, 8, 8, 4, 1, 2)
-QSIM_P2CFGM(float, 2048, 8, 8, 8, 8, 4, 1, 2, 2)
-QSIM_P2CFGM(float, 2048, 9, 16, 16, 8, 1, 1, 4, 2)
-QSIM_P2CFGM(float, 2048, 10, 8, 8, 8, 4, 1, 1, 4)
-QSIM_P2CFGM(float, 2048, 11, 16, 16, 8, 1, 2, 2, 2)
-QSIM_P2CFGS(float, 2048, 12, 8, 8, 8, 4, 1, 4, 1, 2)
-QSIM_P2CFGS(float, 2048, 13, 8, 8, 8, 4, 1, 2, 1, 4)
-QSIM_P2CFGS(float, 2048, 14, 16, 16, 8, 1, 1, 2, 2, 2)
-QSIM_P2CFGS(float, 2048, 15, 16, 16, 8, 1, 2, 4, 1, 2)
 QSIM_P2CFG(float, 4096, 0, 16, 16, 16, 0, 0)
 QSIM_P2CFG4(float, 8192, 0, 16, 8, 8, 8, 0, 0)
 QSIM_P2CFG(float, 16384, 0, 16, 32, 32, 0, 0)  // 1D split jobs only (32 per thread: 512 threads, 256 registers)
 #undef QSIM_P2CFG
 #undef QSIM_P2CFG4
-#undef QSIM_P2CFGM
-#undef QSIM_P2CFGS
 
 __host__ __device__ constexpr int pow2floor(int x) { return x <= 1 ? 1 : 2 * pow2floor(x / 2); }
 
 // Lines per workgroup: rows (position-fast) fill ~256 threads; strided axes
 // (line-fast) take 128 B of adjacent lines per row of the tile; both kept
 // within 96 KiB of LDS and a power of two, unless the variant fixes them.
-// L counts line slots of Line::TPL threads; each slot holds M lines.
 template <typename T, int N, bool LF, int V = 0>
 struct P2Launch {
     using Cfg = P2Cfg<T, N, V>;
     using Line = typename Cfg::Line;
-    static constexpr int M = Cfg::ML;
-    static constexpr int SEQ = Cfg::SQ;
     static constexpr int LF_LINES = 128 / (int)sizeof(cx<T>);
     static constexpr int ROWS = cmax(1, 256 / Line::TPL);
     static constexpr int L_WANT = LF ? LF_LINES : ROWS;
     static constexpr int LS_MAX = Line::LS_RAW + 15;
-    static constexpr int L_LDS = cmax(1, (96 * 1024) / (LS_MAX * (int)sizeof(cx<T>) * M));
+    static constexpr int L_LDS = cmax(1, (96 * 1024) / (LS_MAX * (int)sizeof(cx<T>)));
     static constexpr int L_FIXED = LF ? Cfg::COL_L : Cfg::ROW_L;
     // never more lines than the 160 KiB of LDS one workgroup can hold
-    static constexpr int L_CAP =
-        pow2floor(cmax(1, (160 * 1024) / (LS_MAX * (int)sizeof(cx<T>) * M)));
+    static constexpr int L_CAP = pow2floor(cmax(1, (160 * 1024) / (LS_MAX * (int)sizeof(cx<T>))));
     static constexpr int L = cmin(L_CAP, L_FIXED > 0 ? L_FIXED : pow2floor(cmin(L_WANT, L_LDS)));
     static constexpr int THREADS = L * Line::TPL;
-    // lines per group; a workgroup transforms SEQ groups
-    static constexpr int LINES = L * M;
     // LDS stride between the lines of a workgroup (in elements).  Line-fast
     // tiles put lane i on line i % L, element i / L: a stride of 16/L (mod 16)
     // sends the lanes of one 16-lane group to 16 different 16-byte slots;
@@ -154,8 +102,7 @@ struct P2Launch {
     static constexpr int TARGET = LF ? (L >= 16 ? 1 : 16 / L) : 1;
     static constexpr int LSTR =
         Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
-    static constexpr size_t LDS_BYTES = (size_t)LINES * LSTR * sizeof(cx<T>);
-    static_assert(LDS_BYTES <= 160 * 1024, "workgroup LDS beyond 160 KiB");
+    static constexpr size_t LDS_BYTES = (size_t)L * LSTR * sizeof(cx<T>);
 };
 
 // Plain transform of `nlines` lines of length N with stride S (in may be out).

@@ -62,12 +62,6 @@ __device__ __forceinline__ cx<TO> conv(cx<TI> a)
 }
 
 // ---------------------------------------------------------------- 2D sweeps
-// Each line slot (Line::TPL threads) holds M adjacent lines at a time
-// (M = 1, 2, 4): one twiddle build serves all of them.  A MID row sweep or a
-// column sweep slot transforms SEQ such groups one after another and issues
-// the loads of group q+1 before it transforms group q, so a workgroup's HBM
-// traffic overlaps its own FFT work instead of every workgroup loading,
-// transforming and storing in step.
 template <typename T, int N, int MODE, int V>
 __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
     k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
@@ -75,147 +69,84 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
 {
     using LP = P2Launch<T, N, false, V>;
     using P = typename LP::Line;
-    constexpr int L = LP::L, M = LP::M, E = P::E;
-    constexpr int SEQ = MODE == SW_MID ? LP::SEQ : 1;
+    constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const SplitDev &J = jobs[blockIdx.y];
     const int tid = threadIdx.x;
     const int l = tid / P::TPL, t = tid % P::TPL;
-    // first line of this slot; group q, line m is g0 + q*M + m
-    const size_t g0 = ((size_t)blockIdx.x * L + l) * (SEQ * M);
+    const size_t g = (size_t)blockIdx.x * L + l;
+    const bool active = g < nrows;
+    const size_t base = (active ? g : 0) * N;
     const c128 *__restrict__ ev = J.ev[iev];
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
     c128 *psi = J.psi;
-    cx<T> *ldsl = lds + l * M * LP::LSTR;
     constexpr int RA = P::RA_, RZ = P::RLAST;
-    cx<T> v[M * E];
+    cx<T> v[P::E];
     if constexpr (MODE == SW_FIRST) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const bool act = g0 + m < nrows;
-            const size_t base = (act ? g0 + m : 0) * N;
+        for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
-            for (int u = 0; u < E / RA; ++u)
-#pragma unroll
-                for (int r = 0; r < RA; ++r) {
-                    const size_t i = base + P::template in_pos<RA>(t, u, r);
-                    v[m * E + u * RA + r] =
-                        act ? apply(conv<T>(ld_g(psi + i)), ld_g(ev + i)) : mk<T>(0, 0);
-                }
-        }
-        P::template forward<M>(v, t, ldsl, tw, LP::LSTR);
-#pragma unroll
-        for (int m = 0; m < M; ++m)
-            if (g0 + m < nrows) {
-#pragma unroll
-                for (int u = 0; u < E / RZ; ++u)
-#pragma unroll
-                    for (int r = 0; r < RZ; ++r)
-                        store_nt(&work[(g0 + m) * N + P::template in_pos<RZ>(t, u, r)],
-                                 v[m * E + u * RZ + r]);
+            for (int r = 0; r < RA; ++r) {
+                const int pos = P::template in_pos<RA>(t, u, r);
+                v[u * RA + r] =
+                    active ? apply(conv<T>(ld_g(psi + base + pos)), ld_g(ev + base + pos))
+                           : mk<T>(0, 0);
             }
-        return;
     } else {
-        // the potential factors do not depend on the transform: their loads
-        // are issued with the group's data so they land while the inverse
-        // FFT runs
-        constexpr bool PREFETCH = E * M <= (SEQ > 1 ? 8 : 16);
-        c128 evp[PREFETCH ? E * M : 1];
-        auto fetch = [&](int q, cx<T> *vv, c128 *ee) {
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                const size_t g = g0 + (size_t)q * M + m;
-                const bool act = g < nrows;
-                const size_t base = (act ? g : 0) * N;
+        for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
-                for (int u = 0; u < E / RZ; ++u)
+            for (int r = 0; r < RZ; ++r)
+                v[u * RZ + r] =
+                    active ? ld_g(work + base + P::template in_pos<RZ>(t, u, r)) : mk<T>(0, 0);
+        // the potential factors do not depend on the transform: issue their
+        // loads first so they land while the inverse FFT runs
+        constexpr bool PREFETCH = P::E <= 16;
+        c128 evp[PREFETCH ? P::E : 1];
+        if constexpr (PREFETCH) {
 #pragma unroll
-                    for (int r = 0; r < RZ; ++r)
-                        vv[m * E + u * RZ + r] =
-                            act ? ld_g(work + base + P::template in_pos<RZ>(t, u, r))
-                                : mk<T>(0, 0);
+            for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                for (int r = 0; r < RA; ++r)
+                    evp[u * RA + r] = active ? ld_g(ev + base + P::template in_pos<RA>(t, u, r))
+                                             : mk(0.0, 0.0);
+        }
+        P::inverse(v, t, lds + l * LP::LSTR, tw);
+#pragma unroll
+        for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+            for (int r = 0; r < RA; ++r) {
+                const int pos = P::template in_pos<RA>(t, u, r);
                 if constexpr (PREFETCH) {
-#pragma unroll
-                    for (int u = 0; u < E / RA; ++u)
-#pragma unroll
-                        for (int r = 0; r < RA; ++r)
-                            ee[m * E + u * RA + r] =
-                                act ? ld_g(ev + base + P::template in_pos<RA>(t, u, r))
-                                    : mk(0.0, 0.0);
+                    v[u * RA + r] = apply(v[u * RA + r], evp[u * RA + r]);
+                } else {
+                    if (active) v[u * RA + r] = apply(v[u * RA + r], ld_g(ev + base + pos));
                 }
             }
-        };
-        fetch(0, v, evp);
+    }
+    if constexpr (MODE == SW_LAST) {
+        if (active) {
 #pragma unroll
-        for (int q = 0; q < SEQ; ++q) {
-            cx<T> vn[M * E];
-            c128 en[PREFETCH ? M * E : 1];
-            if constexpr (SEQ > 1)
-                if (q + 1 < SEQ) fetch(q + 1, vn, en);
-            P::template inverse<M>(v, t, ldsl, tw, LP::LSTR);
+            for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                const size_t g = g0 + (size_t)q * M + m;
-                const bool act = g < nrows;
-                const size_t base = (act ? g : 0) * N;
+                for (int r = 0; r < RA; ++r)
+                    store_nt(&psi[base + P::template in_pos<RA>(t, u, r)],
+                             conv<double>(v[u * RA + r]));
+        }
+    } else {
+        P::forward(v, t, lds + l * LP::LSTR, tw);
+        if (active) {
 #pragma unroll
-                for (int u = 0; u < E / RA; ++u)
+            for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
-                    for (int r = 0; r < RA; ++r) {
-                        cx<T> &x = v[m * E + u * RA + r];
-                        if constexpr (PREFETCH) {
-                            x = apply(x, evp[m * E + u * RA + r]);
-                        } else {
-                            if (act)
-                                x = apply(x, ld_g(ev + base + P::template in_pos<RA>(t, u, r)));
-                        }
-                    }
-            }
-            if constexpr (MODE == SW_LAST) {
-#pragma unroll
-                for (int m = 0; m < M; ++m)
-                    if (g0 + m < nrows) {
-#pragma unroll
-                        for (int u = 0; u < E / RA; ++u)
-#pragma unroll
-                            for (int r = 0; r < RA; ++r)
-                                store_nt(&psi[(g0 + m) * N + P::template in_pos<RA>(t, u, r)],
-                                         conv<double>(v[m * E + u * RA + r]));
-                    }
-            } else {
-                P::template forward<M>(v, t, ldsl, tw, LP::LSTR);
-#pragma unroll
-                for (int m = 0; m < M; ++m) {
-                    const size_t g = g0 + (size_t)q * M + m;
-                    if (g < nrows) {
-#pragma unroll
-                        for (int u = 0; u < E / RZ; ++u)
-#pragma unroll
-                            for (int r = 0; r < RZ; ++r)
-                                store_nt(&work[g * N + P::template in_pos<RZ>(t, u, r)],
-                                         v[m * E + u * RZ + r]);
-                    }
-                }
-            }
-            if constexpr (SEQ > 1) {
-                if (q + 1 < SEQ) {
-#pragma unroll
-                    for (int i = 0; i < M * E; ++i) v[i] = vn[i];
-                    if constexpr (PREFETCH) {
-#pragma unroll
-                        for (int i = 0; i < M * E; ++i) evp[i] = en[i];
-                    }
-                }
-            }
+                for (int r = 0; r < RZ; ++r)
+                    store_nt(&work[base + P::template in_pos<RZ>(t, u, r)], v[u * RZ + r]);
         }
     }
 }
 
-// Column sweep over ncols columns of an nx(=N) x ncols array.  A workgroup
-// takes SEQ groups of L*M adjacent columns; in group q, slot l holds columns
-// l, l + L, ... of the group, so every load and store instruction covers L
-// adjacent columns.
+// Column sweep over ncols columns of an nx(=N) x ncols array.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
     k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
@@ -223,78 +154,42 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
 {
     using LP = P2Launch<T, N, true, V>;
     using P = typename LP::Line;
-    constexpr int L = LP::L, M = LP::M, E = P::E, SEQ = LP::SEQ;
+    constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const SplitDev &J = jobs[blockIdx.y];
     const int tid = threadIdx.x;
     const int l = tid % L, t = tid / L;
-    const size_t jb = xcd_tile(blockIdx.x, gridDim.x) * (L * M * SEQ) + l;
+    const size_t j = xcd_tile(blockIdx.x, gridDim.x) * L + l;
+    const bool active = j < ncols;
+    const size_t c = active ? j : 0;
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
     const c128 *__restrict__ kx = J.kx[ik];
-    const c128 *__restrict__ ky = J.ky[ik];
-    // line m of slot l sits at LDS line m*L + l (the stride pattern of the
-    // one-line tiles holds for each m)
-    cx<T> *ldsl = lds + l * LP::LSTR;
-    constexpr int LDM = L * LP::LSTR;
+    const c128 kyj = ld_g(J.ky[ik] + c);
     constexpr int RA = P::RA_, RZ = P::RLAST;
-    auto col = [&](int q, int m) { return jb + (size_t)q * (L * M) + (size_t)m * L; };
-    auto fetch = [&](int q, cx<T> *vv, c128 *kk) {
+    cx<T> v[P::E];
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const size_t j = col(q, m);
-            const bool act = j < ncols;
-            const size_t c = act ? j : 0;
-            kk[m] = ld_g(ky + c);
+    for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
-            for (int u = 0; u < E / RA; ++u)
+        for (int r = 0; r < RA; ++r)
+            v[u * RA + r] =
+                active ? ld_g(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
+                       : mk<T>(0, 0);
+    P::forward(v, t, lds + l * LP::LSTR, tw);
 #pragma unroll
-                for (int r = 0; r < RA; ++r)
-                    vv[m * E + u * RA + r] =
-                        act ? ld_g(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
-                            : mk<T>(0, 0);
+    for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) {
+            const int k = P::template in_pos<RZ>(t, u, r);
+            v[u * RZ + r] = apply(v[u * RZ + r], ld_g(kx + k) * kyj);
         }
-    };
-    cx<T> v[M * E];
-    c128 kyj[M];
-    fetch(0, v, kyj);
+    P::inverse(v, t, lds + l * LP::LSTR, tw);
+    if (active) {
 #pragma unroll
-    for (int q = 0; q < SEQ; ++q) {
-        cx<T> vn[M * E];
-        c128 kn[M];
-        if constexpr (SEQ > 1)
-            if (q + 1 < SEQ) fetch(q + 1, vn, kn);
-        P::template forward<M>(v, t, ldsl, tw, LDM);
+        for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
-        for (int u = 0; u < E / RZ; ++u)
-#pragma unroll
-            for (int r = 0; r < RZ; ++r) {
-                const c128 kk = ld_g(kx + P::template in_pos<RZ>(t, u, r));
-#pragma unroll
-                for (int m = 0; m < M; ++m)
-                    v[m * E + u * RZ + r] = apply(v[m * E + u * RZ + r], kk * kyj[m]);
-            }
-        P::template inverse<M>(v, t, ldsl, tw, LDM);
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const size_t j = col(q, m);
-            if (j < ncols) {
-#pragma unroll
-                for (int u = 0; u < E / RA; ++u)
-#pragma unroll
-                    for (int r = 0; r < RA; ++r)
-                        store_nt(&work[j + (size_t)P::template in_pos<RA>(t, u, r) * ncols],
-                                 v[m * E + u * RA + r]);
-            }
-        }
-        if constexpr (SEQ > 1) {
-            if (q + 1 < SEQ) {
-#pragma unroll
-                for (int i = 0; i < M * E; ++i) v[i] = vn[i];
-#pragma unroll
-                for (int m = 0; m < M; ++m) kyj[m] = kn[m];
-            }
-        }
+            for (int r = 0; r < RA; ++r)
+                store_nt(&work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols], v[u * RA + r]);
     }
 }
 
@@ -486,9 +381,7 @@ int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const voi
     using LP = P2Launch<T, N, false, V>;
     auto fn = k_row_sweep<T, N, MODE, V>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
-    // the FIRST and LAST sweeps transform one group per slot
-    constexpr size_t per_wg = (size_t)LP::LINES * (MODE == SW_MID ? LP::SEQ : 1);
-    dim3 grid((unsigned)((nrows + per_wg - 1) / per_wg), (unsigned)njobs);
+    dim3 grid((unsigned)((nrows + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, nrows, iev,
                        (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
@@ -502,8 +395,7 @@ int launch_col(const SplitDev *jobs, int njobs, size_t ncols, int ik, const void
     using LP = P2Launch<T, N, true, V>;
     auto fn = k_col_sweep<T, N, V>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
-    constexpr size_t per_wg = (size_t)LP::LINES * LP::SEQ;
-    dim3 grid((unsigned)((ncols + per_wg - 1) / per_wg), (unsigned)njobs);
+    dim3 grid((unsigned)((ncols + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, ncols, ik,
                        (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
@@ -544,7 +436,7 @@ int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps
 
 // Launch variants: lengths 1024 and 2048 have the candidates 0..kVariants-1
 // (P2Cfg), every other length only variant 0.
-constexpr int kVariants = 16;
+constexpr int kVariants = 8;
 
 bool has_variants(size_t n) { return n == 1024 || n == 2048; }
 
@@ -561,14 +453,6 @@ int row_v(int var, const SplitDev *jobs, int njobs, size_t nrows, int iev, const
             case 5: return launch_row<T, NN, MODE, 5>(jobs, njobs, nrows, iev, tw, s);
             case 6: return launch_row<T, NN, MODE, 6>(jobs, njobs, nrows, iev, tw, s);
             case 7: return launch_row<T, NN, MODE, 7>(jobs, njobs, nrows, iev, tw, s);
-            case 8: return launch_row<T, NN, MODE, 8>(jobs, njobs, nrows, iev, tw, s);
-            case 9: return launch_row<T, NN, MODE, 9>(jobs, njobs, nrows, iev, tw, s);
-            case 10: return launch_row<T, NN, MODE, 10>(jobs, njobs, nrows, iev, tw, s);
-            case 11: return launch_row<T, NN, MODE, 11>(jobs, njobs, nrows, iev, tw, s);
-            case 12: return launch_row<T, NN, MODE, 12>(jobs, njobs, nrows, iev, tw, s);
-            case 13: return launch_row<T, NN, MODE, 13>(jobs, njobs, nrows, iev, tw, s);
-            case 14: return launch_row<T, NN, MODE, 14>(jobs, njobs, nrows, iev, tw, s);
-            case 15: return launch_row<T, NN, MODE, 15>(jobs, njobs, nrows, iev, tw, s);
             default: break;
         }
     }
@@ -588,14 +472,6 @@ int col_v(int var, const SplitDev *jobs, int njobs, size_t ncols, int ik, const 
             case 5: return launch_col<T, NN, 5>(jobs, njobs, ncols, ik, tw, s);
             case 6: return launch_col<T, NN, 6>(jobs, njobs, ncols, ik, tw, s);
             case 7: return launch_col<T, NN, 7>(jobs, njobs, ncols, ik, tw, s);
-            case 8: return launch_col<T, NN, 8>(jobs, njobs, ncols, ik, tw, s);
-            case 9: return launch_col<T, NN, 9>(jobs, njobs, ncols, ik, tw, s);
-            case 10: return launch_col<T, NN, 10>(jobs, njobs, ncols, ik, tw, s);
-            case 11: return launch_col<T, NN, 11>(jobs, njobs, ncols, ik, tw, s);
-            case 12: return launch_col<T, NN, 12>(jobs, njobs, ncols, ik, tw, s);
-            case 13: return launch_col<T, NN, 13>(jobs, njobs, ncols, ik, tw, s);
-            case 14: return launch_col<T, NN, 14>(jobs, njobs, ncols, ik, tw, s);
-            case 15: return launch_col<T, NN, 15>(jobs, njobs, ncols, ik, tw, s);
             default: break;
         }
     }
@@ -1039,30 +915,41 @@ int qsim_hip_split_set_variants(qsim_split_job *j, int row_variant, int col_vari
     return QSIM_OK;
 }
 
-int qsim_hip_split_autotune(qsim_split_job *j, int reps, int *row_variant, int *col_variant,
-                            double *row_us, double *col_us, qsim_stream_t stream)
+}  // extern "C"
+
+namespace {
+
+// Times the sweep variants on copies of the jobs' working states (the jobs'
+// own state is untouched), with all njobs jobs in each launch as a batch runs
+// them, and sets the fastest pair on every job.
+int autotune_jobs(qsim_split_job *const *jobs, int njobs, int reps, int *row_variant,
+                  int *col_variant, double *row_us, double *col_us, hipStream_t s)
 {
-    QSIM_CHECK(j, "null job");
-    QSIM_CHECK(reps >= 1, "reps must be >= 1");
-    if (!j->fused || j->d.dim != 2) {
-        if (row_variant) *row_variant = 0;
-        if (col_variant) *col_variant = 0;
-        return QSIM_OK;
-    }
-    hipStream_t s = as_stream(stream);
+    qsim_split_job *j = jobs[0];
     const int prec = j->d.precision;
     const size_t esz = prec == 64 ? sizeof(c128) : sizeof(c64);
-    // time on a copy of the working state so the job's psi is untouched
-    void *scratch = nullptr, *tab = nullptr;
-    QSIM_HIP(hipMalloc(&scratch, j->n * esz));
-    hipError_t e = hipMemcpyAsync(scratch, j->work, j->n * esz, hipMemcpyDeviceToDevice, s);
-    SplitDev h = j->host;
-    h.work = scratch;
-    if (e == hipSuccess) e = hipMalloc(&tab, sizeof(SplitDev));
-    if (e == hipSuccess) e = hipMemcpy(tab, &h, sizeof(SplitDev), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void)hipFree(scratch);
+    std::vector<void *> scratch(njobs, nullptr);
+    std::vector<SplitDev> h(njobs);
+    void *tab = nullptr;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < njobs && e == hipSuccess; ++i) {
+        e = hipMalloc(&scratch[i], j->n * esz);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(scratch[i], jobs[i]->work, j->n * esz, hipMemcpyDeviceToDevice, s);
+        h[i] = jobs[i]->host;
+        h[i].work = scratch[i];
+    }
+    if (e == hipSuccess) e = hipMalloc(&tab, sizeof(SplitDev) * njobs);
+    if (e == hipSuccess)
+        e = hipMemcpy(tab, h.data(), sizeof(SplitDev) * njobs, hipMemcpyHostToDevice);
+    auto release = [&]() {
+        for (void *p : scratch)
+            if (p) (void)hipFree(p);
         if (tab) (void)hipFree(tab);
+    };
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        release();
         return hip_error(e, "autotune setup");
     }
     const SplitDev *jt = (const SplitDev *)tab;
@@ -1077,9 +964,10 @@ int qsim_hip_split_autotune(qsim_split_job *j, int reps, int *row_variant, int *
     int var[2] = {j->row_var, j->col_var};
     double best[2] = {1e30, 1e30};
     auto pair = [&](int vr_, int vc_) {
-        int r2 = row_sweep(prec, j->d.ny, SW_MID, vr_, jt, 1, j->d.nx, j->ph_e[0], j->tw_y, s);
+        int r2 =
+            row_sweep(prec, j->d.ny, SW_MID, vr_, jt, njobs, j->d.nx, j->ph_e[0], j->tw_y, s);
         if (r2 == QSIM_OK)
-            r2 = col_sweep(prec, j->d.nx, vc_, jt, 1, j->d.ny, j->ph_k[0], j->tw_x, s);
+            r2 = col_sweep(prec, j->d.nx, vc_, jt, njobs, j->d.ny, j->ph_k[0], j->tw_x, s);
         return r2;
     };
     for (int which = 0; which < 2 && rc == QSIM_OK; ++which) {
@@ -1105,22 +993,63 @@ int qsim_hip_split_autotune(qsim_split_job *j, int reps, int *row_variant, int *
         }
         var[which] = chosen;
     }
-    const int vr = var[0], vc = var[1];
-    const double best_r = best[0], best_c = best[1];
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipStreamSynchronize(s);
-    (void)hipFree(scratch);
-    (void)hipFree(tab);
+    release();
     if (rc != QSIM_OK) return rc;
-    j->row_var = vr;
-    j->col_var = vc;
-    drop_graphs(j);
-    if (row_variant) *row_variant = vr;
-    if (col_variant) *col_variant = vc;
-    if (row_us) *row_us = best_r;
-    if (col_us) *col_us = best_c;
+    for (int i = 0; i < njobs; ++i) {
+        jobs[i]->row_var = var[0];
+        jobs[i]->col_var = var[1];
+        drop_graphs(jobs[i]);
+    }
+    if (row_variant) *row_variant = var[0];
+    if (col_variant) *col_variant = var[1];
+    if (row_us) *row_us = best[0];
+    if (col_us) *col_us = best[1];
     return QSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsim_hip_split_autotune(qsim_split_job *j, int reps, int *row_variant, int *col_variant,
+                            double *row_us, double *col_us, qsim_stream_t stream)
+{
+    QSIM_CHECK(j, "null job");
+    QSIM_CHECK(reps >= 1, "reps must be >= 1");
+    if (!j->fused || j->d.dim != 2) {
+        if (row_variant) *row_variant = 0;
+        if (col_variant) *col_variant = 0;
+        return QSIM_OK;
+    }
+    return autotune_jobs(&j, 1, reps, row_variant, col_variant, row_us, col_us,
+                         as_stream(stream));
+}
+
+int qsim_hip_split_autotune_batched(qsim_split_job *const *jobs, int njobs, int reps,
+                                    int *row_variant, int *col_variant, double *row_us,
+                                    double *col_us, qsim_stream_t stream)
+{
+    QSIM_CHECK(jobs && njobs >= 1, "null job list");
+    QSIM_CHECK(reps >= 1, "reps must be >= 1");
+    qsim_split_job *j0 = jobs[0];
+    QSIM_CHECK(j0, "null job");
+    for (int i = 1; i < njobs; ++i) {
+        const qsim_split_job *ji = jobs[i];
+        QSIM_CHECK(ji && ji->fused == j0->fused && ji->d.dim == j0->d.dim &&
+                       ji->d.nx == j0->d.nx && ji->d.ny == j0->d.ny &&
+                       ji->d.method == j0->d.method && ji->d.precision == j0->d.precision,
+                   "batched autotune: jobs differ in shape, method or precision");
+    }
+    if (!j0->fused || j0->d.dim != 2) {
+        if (row_variant) *row_variant = 0;
+        if (col_variant) *col_variant = 0;
+        return QSIM_OK;
+    }
+    return autotune_jobs(jobs, njobs, reps, row_variant, col_variant, row_us, col_us,
+                         as_stream(stream));
 }
 
 int qsim_hip_split_load_psi(qsim_split_job *j, qsim_stream_t)

@@ -157,6 +157,45 @@ def test_batched_equals_alone():
         assert np.array_equal(a, b.GetPsi())
 
 
+def test_batched_autotune_sets_one_pair_on_every_job(oracle):
+    """the batched autotuner leaves the jobs' states untouched, gives them all
+    the same variants (qsim_hip_split_batch_steps refuses mixed ones) and the
+    batch then matches the oracle"""
+    import ctypes
+    from qusim_amd._native import call
+    psi, V, L0, L1 = gauss2d(64, 1024)
+    evs = [run_2d(psi, V, L0, L1, 0.05, 0, 2, complex_v=True) for _ in range(3)]
+    arr = (ctypes.c_void_p * 3)(*[e._job for e in evs])
+    vr, vc = ctypes.c_int(-1), ctypes.c_int(-1)
+    ru, cu = ctypes.c_double(), ctypes.c_double()
+    call("qsim_hip_split_autotune_batched", arr, 3, 2, ctypes.byref(vr), ctypes.byref(vc),
+         ctypes.byref(ru), ctypes.byref(cu), None)
+    assert 0 <= vr.value and vc.value == 0 and ru.value > 0
+    b = ctypes.c_void_p()
+    call("qsim_hip_split_batch_create", ctypes.byref(b), 3)
+    call("qsim_hip_split_batch_steps", b, arr, 3, 6, None)
+    call("qsim_hip_split_batch_destroy", b)
+    ref = oracle.split2d(psi, V, L0[0], L0[1], L1[0], L1[1], 0.05, 6, method=2, complex_v=True)
+    for e in evs:
+        assert rel_l2(e.GetPsi(), ref) < 1e-12
+
+
+def test_fp32_mixed_1000_steps_within_1e5_of_fp64():
+    """FP32-mixed against the complex128 device path (itself pinned to the
+    oracle) after 1000 steps of the 1024^2-construction problem at 512^2: the
+    phase factors' float-float parts enter before the last rounding, so
+    near-identity factors lose nothing the same way every step"""
+    from qusim_amd.workloads import c2_problem
+    p = c2_problem(n=512)
+    L = (p["x0"], p["x1"])
+    a = run_2d(p["psi"], p["V"], L, L, p["dt"], 1000, 2, complex_v=True).GetPsi()
+    b = run_2d(p["psi"], p["V"], L, L, p["dt"], 1000, 2, precision="mixed32",
+               complex_v=True).GetPsi()
+    err = rel_l2(b, a)
+    print("fp32-mixed 512^2 x1000 rel L2 vs complex128", err)
+    assert err < 1e-5
+
+
 def test_norm_conserved_for_real_potential():
     psi, V, L0, L1 = gauss2d(256, 256)
     e = run_2d(psi, V.real, L0, L1, 0.1, 0, 2)
@@ -183,13 +222,13 @@ def test_1d_fp32_long_lines(n, oracle):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("variant", range(16))
+@pytest.mark.parametrize("variant", range(8))
 @pytest.mark.parametrize("shape", [(1024, 64), (64, 1024), (2048, 64), (64, 2048)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_launch_variants_match_oracle(shape, variant, oracle):
     """every row / column launch variant of the 1024 and 2048 sweeps (radix
-    plans of three and four passes, one to four lines per thread) against the
-    oracle (pipelined forms included); FP32-mixed against the FP64 device result"""
+    plans of three and four passes) against the oracle; FP32-mixed against
+    the FP64 device result"""
     from qusim_amd._native import call
     psi, V, L0, L1 = gauss2d(*shape)
     outs = {}
