@@ -234,7 +234,9 @@ int qsim_hip_split_steps_timed(qsim_split_job *job, size_t nsteps, double *kerne
 int qsim_hip_split_use_graph(qsim_split_job *job, int enable);
 /* Launch variants of the 2D sweep kernels (radix set, lines per workgroup):
  * 0..7 for axis lengths 1024 and 2048 (6 and 7 run four radix passes with
- * 8 elements per thread), only 0 otherwise. */
+ * 8 elements per thread), only 0 otherwise.  A new job starts with a fixed
+ * pair measured fastest on MI355X for its lengths (FP64 1024: 4/4, FP32
+ * 1024: 7/7, FP64 2048: 7/5, FP32 2048: 7/3). */
 int qsim_hip_split_set_variants(qsim_split_job *job, int row_variant, int col_variant);
 /* Autotuner: times every row variant, then every column variant, each in reps
  * alternating (row, column) sweep pairs on a copy of the state, keeps the

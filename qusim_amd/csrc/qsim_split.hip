@@ -440,6 +440,18 @@ constexpr int kVariants = 8;
 
 bool has_variants(size_t n) { return n == 1024 || n == 2048; }
 
+// Variants a new job starts with: the pairs the autotuner picked most often
+// on MI355X for single and batched jobs of these lengths (row variant by the
+// row length ny, column variant by the column length nx).  Fixed, so jobs that
+// are never tuned (request loops, batches) still take a fast pair and two
+// runs of one job always choose the same radix plans.
+int default_variant(int prec, size_t n, bool row)
+{
+    if (n == 1024) return prec == 64 ? 4 : 7;
+    if (n == 2048) return prec == 64 ? (row ? 7 : 5) : (row ? 7 : 3);
+    return 0;
+}
+
 template <typename T, int NN, int MODE>
 int row_v(int var, const SplitDev *jobs, int njobs, size_t nrows, int iev, const void *tw,
           hipStream_t s)
@@ -671,6 +683,10 @@ int create(qsim_split_job *j, const double *Vre, const double *Vim)
     const cd dt(d.dt_re, d.dt_im);
     j->fused = d.boundary == 0 && fused_len_ok(d.nx, d.precision, d.dim) &&
                (d.dim == 1 || fused_len_ok(d.ny, d.precision, d.dim));
+    if (j->fused && d.dim == 2) {
+        j->row_var = default_variant(d.precision, d.ny, true);
+        j->col_var = default_variant(d.precision, d.nx, false);
+    }
     if (!j->fused && d.precision != 64)
         return set_error(QSIM_ERR_UNSUPPORTED,
                          "FP32-mixed split job needs power-of-two axes (2D: 64..8192, 1D: "
