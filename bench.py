@@ -133,6 +133,29 @@ def probe_kernels(rv, cv):
             ("col_sweep", "k_col_sweep<double, 1024, %d>" % cv))
 
 
+def batch_probe(steps, njobs, variants):
+    """the FP32-mixed batched workload alone, for rocprofv3: njobs 1024^2
+    jobs advanced together with fixed launch variants (no autotuner trials in
+    the trace)"""
+    from qusim_amd import _native as Nn
+    from qusim_amd.workloads import c2_problem
+    Nn.init(0)
+    prob = c2_problem()
+    jobs = [make_job(Nn, prob, 32) for _ in range(njobs)]
+    ptrs = [j[0] for j in jobs]
+    for p_ in ptrs:
+        Nn.call("qsim_hip_split_set_variants", p_, variants[0], variants[1])
+    arr = (ctypes.c_void_p * njobs)(*ptrs)
+    b = ctypes.c_void_p()
+    Nn.call("qsim_hip_split_batch_create", ctypes.byref(b), njobs)
+    Nn.call("qsim_hip_split_batch_steps", b, arr, njobs, steps, None)
+    Nn.synchronize()
+    Nn.call("qsim_hip_split_batch_destroy", b)
+    for p_ in ptrs:
+        Nn.call("qsim_hip_split_destroy", p_)
+    log("batch probe: %d FP32-mixed 1024^2 jobs, %d steps, variants %s" % (njobs, steps, variants))
+
+
 def traffic_probe(steps):
     """Workload of the rocprofv3 --pmc passes: 3 calibration copies of
     PROBE_COPY_N complex128 (known bytes), then `steps` C2 steps with fixed
@@ -413,6 +436,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run the workload of the rocprofv3 --pmc passes and exit")
     ap.add_argument("--probe-steps", type=int, default=6)
+    ap.add_argument("--batch-probe", type=int, metavar="NJOBS",
+                    help="run only NJOBS batched FP32-mixed 1024^2 jobs (for rocprofv3) and exit")
+    ap.add_argument("--probe-variants", type=int, nargs=2, default=(7, 7))
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc outputs (CPU)")
     args = ap.parse_args()
@@ -421,6 +447,9 @@ def main():
         return
     if args.traffic_probe:
         traffic_probe(args.probe_steps)
+        return
+    if args.batch_probe:
+        batch_probe(args.probe_steps, args.batch_probe, tuple(args.probe_variants))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
