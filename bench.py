@@ -133,14 +133,14 @@ def probe_kernels(rv, cv):
             ("col_sweep", "k_col_sweep<double, 1024, %d>" % cv))
 
 
-def batch_probe(steps, njobs, variants):
-    """the FP32-mixed batched workload alone, for rocprofv3: njobs 1024^2
-    jobs advanced together with fixed launch variants (no autotuner trials in
-    the trace)"""
+def batch_probe(steps, njobs, variants, n=1024):
+    """the FP32-mixed batched workload alone, for rocprofv3: njobs n^2 jobs
+    (the 1024^2 construction, or the 2048^2 dt = 0.1 one) advanced together
+    with fixed launch variants (no autotuner trials in the trace)"""
     from qusim_amd import _native as Nn
-    from qusim_amd.workloads import c2_problem
+    from qusim_amd.workloads import c2_problem, c3_problem
     Nn.init(0)
-    prob = c2_problem()
+    prob = c2_problem() if n == 1024 else c3_problem(n)
     jobs = [make_job(Nn, prob, 32) for _ in range(njobs)]
     ptrs = [j[0] for j in jobs]
     for p_ in ptrs:
@@ -153,7 +153,7 @@ def batch_probe(steps, njobs, variants):
     Nn.call("qsim_hip_split_batch_destroy", b)
     for p_ in ptrs:
         Nn.call("qsim_hip_split_destroy", p_)
-    log("batch probe: %d FP32-mixed 1024^2 jobs, %d steps, variants %s" % (njobs, steps, variants))
+    log("batch probe: %d FP32-mixed %d^2 jobs, %d steps, variants %s" % (njobs, n, steps, variants))
 
 
 def traffic_probe(steps):
@@ -327,6 +327,9 @@ def fp32_section(steps_err, steps_rate, fp64_plain_1024):
                       "fp32_batched": b3, "fp64_single": rate64,
                       "speedup_single_vs_fp64": rate32 / rate64,
                       "speedup_batched_vs_fp64": b3["value"] / rate64,
+                      # against the plain FP64 rate of the 1024^2 headline job
+                      "speedup_single_vs_c2_fp64": rate32 / fp64_plain_1024,
+                      "speedup_batched_vs_c2_fp64": b3["value"] / fp64_plain_1024,
                       "error": e3, "dominant_kernel": dom, "kernels": per},
     }
 
@@ -439,6 +442,7 @@ def main():
     ap.add_argument("--batch-probe", type=int, metavar="NJOBS",
                     help="run only NJOBS batched FP32-mixed 1024^2 jobs (for rocprofv3) and exit")
     ap.add_argument("--probe-variants", type=int, nargs=2, default=(7, 7))
+    ap.add_argument("--probe-n", type=int, default=1024, choices=(1024, 2048))
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc outputs (CPU)")
     args = ap.parse_args()
@@ -449,7 +453,7 @@ def main():
         traffic_probe(args.probe_steps)
         return
     if args.batch_probe:
-        batch_probe(args.probe_steps, args.batch_probe, tuple(args.probe_variants))
+        batch_probe(args.probe_steps, args.batch_probe, tuple(args.probe_variants), args.probe_n)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
