@@ -1107,22 +1107,22 @@ int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps
         for (int i = 0; i < njobs; ++i) QSIM_TRY(qsim_hip_split_steps(jobs[i], nsteps, stream));
         return QSIM_OK;
     }
-    // one launch per sweep for all jobs: gather their descriptors
-    static thread_local SplitDev *table = nullptr;
-    static thread_local int table_cap = 0;
-    if (table_cap < njobs) {
-        if (table) (void)hipFree(table);
-        table = nullptr;
-        table_cap = 0;
-        QSIM_HIP(hipMalloc(&table, sizeof(SplitDev) * njobs));
-        table_cap = njobs;
-    }
+    // one launch per sweep for all jobs: their descriptors in a table on the
+    // current device, owned by this call (the call is synchronous; request
+    // loops that reuse a table use qsim_split_batch)
     std::vector<SplitDev> h(njobs);
     for (int i = 0; i < njobs; ++i) h[i] = jobs[i]->host;
-    QSIM_HIP(hipMemcpyAsync(table, h.data(), sizeof(SplitDev) * njobs, hipMemcpyHostToDevice, s));
-    QSIM_TRY(fused_steps(j0, table, njobs, nsteps, s));
-    // the host vector must outlive the async copy
-    QSIM_HIP(hipStreamSynchronize(s));
+    SplitDev *table = nullptr;
+    QSIM_HIP(hipMalloc(&table, sizeof(SplitDev) * njobs));
+    hipError_t e = hipMemcpyAsync(table, h.data(), sizeof(SplitDev) * njobs,
+                                  hipMemcpyHostToDevice, s);
+    int rc = e == hipSuccess ? fused_steps(j0, table, njobs, nsteps, s)
+                             : hip_error(e, "batched steps: table upload");
+    // the table and the host vector must outlive the queued work
+    e = hipStreamSynchronize(s);
+    (void)hipFree(table);
+    if (rc != QSIM_OK) return rc;
+    if (e != hipSuccess) return hip_error(e, "batched steps");
     return QSIM_OK;
 }
 
