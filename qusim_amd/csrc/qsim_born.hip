@@ -47,6 +47,7 @@ struct BornDev {
     const c128 *pw[3];     // plane-wave factor per axis: exp(i k_a x_a)
     const double *ab[3];   // absorbing-layer Im V per axis (zeros when naive)
     double E, eps, slow, inv_n;
+    double inv_eps;        // 1 / eps (0 when eps is 0)
     double a[3];           // (2 pi hbar / (d n))^2 / 2m per axis
     double *resid;         // per-block residual partial sums [2 * nblocks]
 };
@@ -66,17 +67,46 @@ __device__ __forceinline__ c128 psi0_at(const BornDev &J, int dim, size_t i, siz
                     : ld_g(J.pw[0] + i) * ld_g(J.pw[1] + j);
 }
 
+// gamma of op_Vellekoop / op_Hao1 (src/DeviceTemplate.h:1020-1052) with the
+// divisions by eps done as products with 1/eps: within an ulp of the
+// quotient, without two FP64 divide sequences per point in the fused sweeps
+// (op_Hao2 keeps its complex division)
 template <int PRE>
-__device__ __forceinline__ c128 update(c128 dpsi, c128 t, c128 v, double slow, double eps)
+__device__ __forceinline__ c128 gamma_fused(c128 v, double slow, double eps, double inv_eps)
+{
+    if constexpr (PRE == PRE_VELLEKOOP) {
+        const c128 iv = imul(v);
+        return mk(1. - iv.x * inv_eps, -iv.y * inv_eps) * slow;
+    } else if constexpr (PRE == PRE_HAO1) {
+        const c128 iv = imul(mk(v.x, -v.y));
+        return mk(1. - iv.x * inv_eps, -iv.y * inv_eps) * slow;
+    } else {
+        return born_gamma<PRE>(v, slow, eps);
+    }
+}
+
+template <int PRE>
+__device__ __forceinline__ c128 update(c128 dpsi, c128 t, c128 v, double slow, double eps,
+                                       double inv_eps)
 {
     if constexpr (PRE == PRE_NAIVE) {
         return t;
     } else if constexpr (PRE == PRE_IDENTITY) {
         return slow == 1. ? t : dpsi * (1 - slow) + t * slow;
     } else {
-        c128 g = born_gamma<PRE>(v, slow, eps);
+        c128 g = gamma_fused<PRE>(v, slow, eps, inv_eps);
         return mk(1. - g.x, -g.y) * dpsi + g * t;
     }
+}
+
+// a * G0 with G0 = 1 / (w.x + i w.y) (op_G02D / op_G03D,
+// src/DeviceTemplate.h:905-976): a * conj(w) / |w|^2 with one FP64 divide.
+// |w|^2 >= eps^2 > 0 and is at most a few thousand on these grids, so the
+// unscaled form neither overflows nor underflows.
+__device__ __forceinline__ c128 mul_g0(c128 a, double wr, double wi)
+{
+    const double inv = 1.0 / (wr * wr + wi * wi);
+    return mk((a.x * wr + a.y * wi) * inv, (a.y * wr - a.x * wi) * inv);
 }
 
 template <int PRE>
@@ -188,7 +218,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
             c128 d = cvt<double>(dd[u * RA + r]);
             if constexpr (MODE != BA_START) {
                 const c128 tn = cvt<double>(v[u * RA + r]) * J.inv_n;
-                const c128 nd = update<PRE>(d, tn, ve, J.slow, J.eps);
+                const c128 nd = update<PRE>(d, tn, ve, J.slow, J.eps, J.inv_eps);
                 const cx<T> nds = cvt<T>(nd);
                 if (active) store_nt(&dp[base + k], nds);
                 const c128 ndr = cvt<double>(nds);
@@ -301,7 +331,7 @@ __global__ void __launch_bounds__(PLANE_THREADS)
         c128 d = cvt<double>(dd[r]);
         if constexpr (MODE != BA_START) {
             const c128 tn = cvt<double>(v[r]) * J.inv_n;
-            const c128 nd = update<PRE>(d, tn, ve, J.slow, J.eps);
+            const c128 nd = update<PRE>(d, tn, ve, J.slow, J.eps, J.inv_eps);
             const c64 nds = cvt<float>(nd);
             store_nt(&dp[pbase + (size_t)j * N + k], nds);
             const c128 ndr = cvt<double>(nds);
@@ -385,7 +415,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
             const double tt = geo.dim == 3
                                   ? fi * fi * J.a[0] + fj * fj * J.a[1] + fk * fk * J.a[2]
                                   : fi * fi * J.a[0] + fj * fj * J.a[1];
-            const c128 z = cdiv(cvt<double>(v[u * RZ + r]), mk(J.E - tt, J.eps));
+            const c128 z = mul_g0(cvt<double>(v[u * RZ + r]), J.E - tt, J.eps);
             v[u * RZ + r] = cvt<T>(z);
         }
     P::inverse(v, t, lds + l * LP::LSTR, tw);
@@ -820,6 +850,7 @@ int create(qsim_born_job *j)
         double eps = d.epsilon;
         if (d.preconditional) QSIM_TRY(epsilon_for(j, e, &eps));
         J.eps = eps;
+        J.inv_eps = eps != 0 ? 1.0 / eps : 0.0;
         void *p;
         QSIM_TRY(balloc(j, &p, j->n * esz));
         J.dpsi = p;
@@ -1010,22 +1041,22 @@ int unfused_iterate(qsim_born_job *j, int max_iter, double tol, hipStream_t s)
             else
                 QSIM_TRY(qsim_hip_g02d((qsim_c128 *)tmp, J.E, eps, J.a[0], J.a[1], geo.n0, geo.n1, s));
             QSIM_TRY(qsim_hip_fft_exec(j->inv, tmp, tmp, s));
-            const double in = J.inv_n, slow = J.slow;
+            const double in = J.inv_n, slow = J.slow, ie = J.inv_eps;
             switch (pre) {
                 case PRE_NAIVE:
-                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_NAIVE>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_NAIVE>(dp[i], tmp[i] * in, ve[i], slow, eps, ie); }));
                     break;
                 case PRE_IDENTITY:
-                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_IDENTITY>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_IDENTITY>(dp[i], tmp[i] * in, ve[i], slow, eps, ie); }));
                     break;
                 case PRE_VELLEKOOP:
-                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_VELLEKOOP>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_VELLEKOOP>(dp[i], tmp[i] * in, ve[i], slow, eps, ie); }));
                     break;
                 case PRE_HAO1:
-                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_HAO1>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_HAO1>(dp[i], tmp[i] * in, ve[i], slow, eps, ie); }));
                     break;
                 default:
-                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_HAO2>(dp[i], tmp[i] * in, ve[i], slow, eps); }));
+                    QSIM_TRY(each_b(n, s, [=] __device__(size_t i) { dp[i] = update<PRE_HAO2>(dp[i], tmp[i] * in, ve[i], slow, eps, ie); }));
                     break;
             }
             j->iters[e] += 1;
