@@ -175,13 +175,24 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
             v[u * RA + r] =
                 active ? ld_g(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
                        : mk<T>(0, 0);
+    // the kinetic factors depend only on the thread's output positions:
+    // issue their loads with the data so they land during the forward FFT
+    constexpr bool PREFETCH = P::E <= 8;
+    c128 kxp[PREFETCH ? P::E : 1];
+    if constexpr (PREFETCH) {
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                kxp[u * RZ + r] = ld_g(kx + P::template in_pos<RZ>(t, u, r));
+    }
     P::forward(v, t, lds + l * LP::LSTR, tw);
 #pragma unroll
     for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
         for (int r = 0; r < RZ; ++r) {
-            const int k = P::template in_pos<RZ>(t, u, r);
-            v[u * RZ + r] = apply(v[u * RZ + r], ld_g(kx + k) * kyj);
+            const c128 kk = PREFETCH ? kxp[u * RZ + r] : ld_g(kx + P::template in_pos<RZ>(t, u, r));
+            v[u * RZ + r] = apply(v[u * RZ + r], kk * kyj);
         }
     P::inverse(v, t, lds + l * LP::LSTR, tw);
     if (active) {
