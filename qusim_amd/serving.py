@@ -134,7 +134,11 @@ def default_quantum(req):
         return 200
     if req.kind == "evo2d":
         return {256: 100, 512: 50}.get(req.n, 25)
-    return 8
+    # Born iterations: the library checks the tolerance every 4 iterations
+    # inside one call, so a turn only bounds how long a worker holds its
+    # stream; every turn boundary costs a finishing half-iteration and a
+    # residual read-back
+    return {256: 200, 512: 100}.get(req.n, 50)
 
 
 # --------------------------------------------------------------- device jobs
