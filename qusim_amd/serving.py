@@ -380,6 +380,8 @@ class Server:
         # finished jobs not yet freed: a free waits for the whole device, so
         # it happens when the lanes are idle or memory is needed
         self._graveyard = []
+        # an exception that ended a Born worker thread
+        self._worker_error = None
 
     # footprint accounting (shared with the Born worker)
     def _reserve(self, fp):
@@ -416,11 +418,12 @@ class Server:
         """Born solves on a stream of this worker's own, each advanced a
         fixed number of iterations per turn; new requests join between
         turns, one per turn, so several workers share the arrivals."""
-        p = ctypes.c_void_p()
-        N.call("qsim_hip_stream_create", ctypes.byref(p))
-        stream = p.value
         active = []
+        stream = None
         try:
+            p = ctypes.c_void_p()
+            N.call("qsim_hip_stream_create", ctypes.byref(p))
+            stream = p.value
             while True:
                 try:
                     item = inbox.get(timeout=0.002) if not active else inbox.get_nowait()
@@ -439,16 +442,25 @@ class Server:
                         self._finish(item.rid, t0)
                         self._release(footprint(item, self.precision))
                 for job in list(active):
-                    job.advance(self.quantum(job.req), stream)
-                    if job.finished():
-                        self.results[job.req.rid] = job.result(job.req.rid in self.keep, stream)
-                        self._finish(job.req.rid, t0)
-                        self._bury(job)
-                        active.remove(job)
+                    try:
+                        job.advance(self.quantum(job.req), stream)
+                        if job.finished():
+                            self.results[job.req.rid] = job.result(job.req.rid in self.keep,
+                                                                   stream)
+                        else:
+                            continue
+                    except Exception as e:  # this request fails; the others go on
+                        self.errors[job.req.rid] = str(e)
+                    self._finish(job.req.rid, t0)
+                    self._bury(job)
+                    active.remove(job)
+        except Exception as e:  # seen by the main loop, which stops the run
+            self._worker_error = e
         finally:
             for job in active:
                 self._bury(job)
-            N.call("qsim_hip_stream_destroy", stream)
+            if stream is not None:
+                N.call("qsim_hip_stream_destroy", stream)
 
     def run(self, requests):
         """Serve `requests`; returns the statistics (see `summary`)."""
@@ -468,6 +480,8 @@ class Server:
 
         try:
             while self._ndone < total:
+                if self._worker_error is not None:
+                    raise RuntimeError("Born worker stopped: %r" % (self._worker_error,))
                 now = time.perf_counter() - t0
                 while pending and (not self.realtime or pending[0].arrival <= now):
                     waiting.append(pending.popleft())

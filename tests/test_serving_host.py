@@ -54,3 +54,61 @@ def test_chunk_schedule_depends_only_on_the_request():
         chunks.append(c)
         done += c
     assert chunks == [50, 50, 50, 23]
+
+
+class _FakeBorn:
+    """Stands in for a device Born solve: request 1 fails on its first turn."""
+
+    def __init__(self, req, precision):
+        self.req, self.done = req, 0
+
+    def advance(self, q, stream):
+        if self.req.rid == 1:
+            raise RuntimeError("planted failure")
+        self.done += q
+
+    def finished(self):
+        return self.done >= self.req.work
+
+    def result(self, keep, stream=None):
+        return {"iterations": self.done, "dsigma": [1.0]}
+
+    def close(self):
+        pass
+
+
+def _host_only(monkeypatch):
+    from qusim_amd import serving
+    monkeypatch.setattr(serving.N, "init", lambda *a, **k: None)
+    monkeypatch.setattr(serving.N, "call", lambda *a, **k: 0)
+    monkeypatch.setattr(serving, "BornJob", _FakeBorn)
+    return serving
+
+
+def test_failed_born_request_is_reported_and_the_rest_complete(monkeypatch):
+    """a solve that raises inside a worker thread fails alone: its error is
+    recorded, the other requests finish, and run() returns"""
+    serving = _host_only(monkeypatch)
+    reqs = [serving.Request(i, "born2d", 256, 200, 0.0, {}) for i in range(4)]
+    s = serving.Server(precision=32, realtime=False, born_workers=2)
+    st = s.run(reqs)
+    assert set(s.errors) == {1} and "planted failure" in s.errors[1]
+    assert set(s.results) == {0, 2, 3}
+    assert st["completed"] == 3 and st["errors"] == 1
+
+
+def test_dead_born_worker_stops_the_run(monkeypatch):
+    """a worker that cannot start (no stream) ends run() with an error
+    instead of leaving its requests waiting forever"""
+    import pytest
+    serving = _host_only(monkeypatch)
+
+    def no_stream(name, *a, **k):
+        if name == "qsim_hip_stream_create":
+            raise RuntimeError("no device")
+        return 0
+
+    monkeypatch.setattr(serving.N, "call", no_stream)
+    reqs = [serving.Request(i, "born2d", 256, 200, 0.0, {}) for i in range(2)]
+    with pytest.raises(RuntimeError, match="Born worker stopped"):
+        serving.Server(precision=32, realtime=False, born_workers=1).run(reqs)
