@@ -515,44 +515,63 @@ def main():
                      "kernels": per},
     }
     Nn.call("qsim_hip_split_destroy", job)
+    def section(key, fn):
+        """a secondary section: an exception is recorded in its field and the
+        headline line is still printed"""
+        try:
+            out[key] = fn()
+            return out[key]
+        except Exception as e:  # noqa: BLE001 -- reported in the JSON line
+            import traceback
+            traceback.print_exc()
+            out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
+            return None
+
     if not args.no_fp32:
         log("FP32-mixed section (2048^2 alone and batched, 1024^2 batched)")
-        f = fp32_section(args.fp32_err_steps, 50, value)
-        out["fp32_mixed"] = f
-        log("FP32-mixed: 1024^2 x16 batched %.3e (%.2fx FP64 1024^2), 2048^2 %.3e alone, "
-            "%.3e x4 batched; psi error after %d steps %.2e (2048^2), %.2e (1024^2)"
-            % (f["value"], f["speedup_vs_fp64"], f["grid_2048"]["fp32_single"],
-               f["grid_2048"]["fp32_batched"]["value"], args.fp32_err_steps,
-               f["grid_2048"]["error"]["psi_rel_l2_vs_fp64"],
-               f["grid_1024"]["error"]["psi_rel_l2_vs_fp64"]))
+        f = section("fp32_mixed", lambda: fp32_section(args.fp32_err_steps, 50, value))
+        if f:
+            log("FP32-mixed: 1024^2 x16 batched %.3e (%.2fx FP64 1024^2), 2048^2 %.3e alone, "
+                "%.3e x4 batched; psi error after %d steps %.2e (2048^2), %.2e (1024^2)"
+                % (f["value"], f["speedup_vs_fp64"], f["grid_2048"]["fp32_single"],
+                   f["grid_2048"]["fp32_batched"]["value"], args.fp32_err_steps,
+                   f["grid_2048"]["error"]["psi_rel_l2_vs_fp64"],
+                   f["grid_1024"]["error"]["psi_rel_l2_vs_fp64"]))
     if not args.no_born:
         log("Born 128^3 x 32 energies section")
-        out["born_3d"] = born_section()
-        log("Born: %.0f energy-iterations/s" % out["born_3d"]["value"])
+        b = section("born_3d", born_section)
+        if b:
+            log("Born: %.0f energy-iterations/s" % b["value"])
     if not args.no_serving:
         log("serving section: %d requests at %.0f/s" % (args.requests, args.rate))
-        out["serving"] = serving_section(args.requests, args.rate, args.hbm_budget_gib)
-        log("serving: %.1f solves/s, p50 %.3f s, p99 %.3f s"
-            % (out["serving"]["solves_per_s"], out["serving"]["latency_p50_s"],
-               out["serving"]["latency_p99_s"]))
+        v = section("serving",
+                    lambda: serving_section(args.requests, args.rate, args.hbm_budget_gib))
+        if v:
+            log("serving: %.1f solves/s, p50 %.3f s, p99 %.3f s"
+                % (v["solves_per_s"], v["latency_p50_s"], v["latency_p99_s"]))
     if not args.no_cpu:
         nt = host_threads()
         log("CPU baseline: %d steps of the oracle on 1 thread, %d on %d threads"
             % (args.cpu_steps, 2 * args.cpu_steps, nt))
-        rate1, secs1 = cpu_baseline(prob, args.cpu_steps, 1)
-        rate, secs = cpu_baseline(prob, 2 * args.cpu_steps, nt)
-        out["cpu_baseline"] = {
-            "value": rate, "unit": "grid-pt-steps/s", "cores": nt, "kind": "port",
-            "sample": "%d O2 steps of the same 1024^2 complex-V workload (%.1f s wall on %d "
-                      "threads), FP64 oracle port of the reference's unfused op order "
-                      "(Mul, FFT2, Mul, IFFT2, Scale) with KISS-style mixed-radix FFTs; "
-                      "FFT lines and pointwise ops threaded, i.e. stronger than the "
-                      "reference's CpuPar, which threads only the pointwise ops"
-                      % (2 * args.cpu_steps, secs, nt),
-            "single_thread": {"value": rate1, "cores": 1,
-                              "sample": "%d steps (%.1f s), the CpuSeq form"
-                                        % (args.cpu_steps, secs1)}}
+        section("cpu_baseline", lambda: cpu_section(prob, args.cpu_steps, nt))
     print(json.dumps(out), flush=True)
+
+
+def cpu_section(prob, steps, nt):
+    """the oracle port timed on this host: `2 * steps` steps on nt threads
+    (the reported baseline) and `steps` on one thread beside it"""
+    rate1, secs1 = cpu_baseline(prob, steps, 1)
+    rate, secs = cpu_baseline(prob, 2 * steps, nt)
+    return {
+        "value": rate, "unit": "grid-pt-steps/s", "cores": nt, "kind": "port",
+        "sample": "%d O2 steps of the same 1024^2 complex-V workload (%.1f s wall on %d "
+                  "threads), FP64 oracle port of the reference's unfused op order "
+                  "(Mul, FFT2, Mul, IFFT2, Scale) with KISS-style mixed-radix FFTs; "
+                  "FFT lines and pointwise ops threaded, i.e. stronger than the "
+                  "reference's CpuPar, which threads only the pointwise ops"
+                  % (2 * steps, secs, nt),
+        "single_thread": {"value": rate1, "cores": 1,
+                          "sample": "%d steps (%.1f s), the CpuSeq form" % (steps, secs1)}}
 
 
 if __name__ == "__main__":
