@@ -466,8 +466,9 @@ def main():
         batch_probe(args.probe_steps, args.batch_probe, tuple(args.probe_variants), args.probe_n)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(every box here has one GPU; see DESIGN.md for how the grid would shard)",
+        print("bench.py measures one GPU and has no multi-GPU path: requests and energies "
+              "would shard across GPUs with no exchange, and one grid beyond a GPU's memory "
+              "as y-slabs with two all-to-all transposes per step",
               file=sys.stderr)
         sys.exit(2)
 
