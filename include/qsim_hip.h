@@ -32,7 +32,7 @@ extern "C" {
 #define QSIM_ERR_INVALID 1      /* bad argument (size, pointer, option)       */
 #define QSIM_ERR_HIP 2          /* HIP runtime error                          */
 #define QSIM_ERR_NOMEM 3        /* device allocation failed                   */
-#define QSIM_ERR_UNSUPPORTED 4  /* valid request this build does not support  */
+#define QSIM_ERR_UNSUPPORTED 4  /* valid request the library does not support */
 
 typedef struct { double re, im; } qsim_c128;
 typedef struct { float re, im; } qsim_c64;
@@ -55,7 +55,7 @@ int qsim_hip_memcpy_h2d(void *dst, const void *host_src, size_t bytes);
 int qsim_hip_memcpy_d2h(void *host_dst, const void *src, size_t bytes);
 int qsim_hip_memcpy_d2d(void *dst, const void *src, size_t bytes, qsim_stream_t stream);
 
-/* Timing with HIP events on a stream (for measurement harnesses). */
+/* Timing with HIP events on a stream (for measurement code). */
 int qsim_hip_event_create(void **ev);
 int qsim_hip_event_destroy(void *ev);
 int qsim_hip_event_record(void *ev, qsim_stream_t stream);

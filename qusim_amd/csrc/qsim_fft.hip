@@ -225,7 +225,7 @@ int prepare_axis(FftAxis *a, size_t n, size_t S, size_t nlines, int precision)
     if (per_line > lds_max)
         return set_error(QSIM_ERR_UNSUPPORTED,
                          "FFT length " + std::to_string(n) +
-                             " exceeds the single-pass LDS limit of this build");
+                             " exceeds the single-pass LDS limit");
     int L = (int)((64 * 1024) / per_line);
     if (L < 1) L = 1;
     if (L > 16) L = 16;
