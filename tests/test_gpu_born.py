@@ -131,3 +131,43 @@ def test_fp32_iterates_error(oracle):
     err = rel_l2(s.GetPsi(), ref)
     print("fp32 Born 64^3 x60 rel L2", err)
     assert err < 1e-5
+
+
+def test_fp32_plane_sweep_matches_oracle(oracle):
+    """3D FP32 jobs with 128 x 128 y-z planes run the plane sweep (both
+    transforms of a plane in one workgroup, END and START launches); several
+    energies per launch, each against its own FP64 oracle run"""
+    from qusim_amd.api import QuPerturbation3D, SolverMethod
+    shape, lo, hi = (64, 128, 128), (-16.0, -32.0, -32.0), (16.0, 32.0, 32.0)
+    V = pot3d(shape, lo, hi)
+    Es = [1.5, 3.0]
+    s = QuPerturbation3D().init(V, lo[0], hi[0], shape[0], lo[1], hi[1], shape[1], lo[2], hi[2],
+                                shape[2], Es, 0.0, 0, 0, 1, SolverMethod.BornSerise, 1.0, 1.0,
+                                opts(1, 1, 0.5, 12, precision="mixed32"))
+    s.Compute()
+    for k, E in enumerate(Es):
+        ref, veff, _, _ = oracle.born(V, lo, hi, E, (0, 0, 1), order=12, preconditional=1,
+                                      preconditioner=1, slow=0.5, threads=8)
+        err = rel_l2(s.GetPsi(k), ref)
+        print("fp32 plane sweep (64,128,128) E=%g x12 rel L2 %.2e" % (E, err))
+        assert err < 1e-5
+        psi0 = oracle.plane_wave(shape, lo, hi, E, (0, 0, 1))
+        xs, _ = oracle.xsection3d(psi0, ref, veff, lo, hi, E, 0.0, 0.0, 1.0)
+        assert abs(s.ComputeXSection(0.0, 0.0, 1.0, k) - xs) < 1e-5 * xs
+
+
+def test_c4_grid_fp64_matches_oracle(oracle):
+    """the full C4 grid (128^3 on [-32,32)^3, seeded Gaussians plus the
+    absorbing layer, Vellekoop, slow 0.5) in complex128 against the oracle"""
+    from qusim_amd.api import QuPerturbation3D, SolverMethod
+    from qusim_amd.workloads import c4_energies, c4_potential
+    V, E = c4_potential(n=128, L=32.0), c4_energies()[0]
+    lo, hi = (-32.0,) * 3, (32.0,) * 3
+    s = QuPerturbation3D().init(V, -32.0, 32.0, 128, -32.0, 32.0, 128, -32.0, 32.0, 128, E, 0.0,
+                                0, 0, 1, SolverMethod.BornSerise, 1.0, 1.0, opts(1, 1, 0.5, 3))
+    s.Compute()
+    ref, _, _, _ = oracle.born(V, lo, hi, E, (0, 0, 1), order=3, preconditional=1,
+                               preconditioner=1, slow=0.5, threads=8)
+    err = rel_l2(s.GetPsi(), ref)
+    print("C4 grid FP64 x3 rel L2 %.2e" % err)
+    assert err < 1e-12
