@@ -11,7 +11,13 @@ the timed region starts.
 Prints one JSON line: grid-pt-steps/s, the per-kernel HBM roofline of the
 dominant kernel (algorithmic bytes / HIP-event-measured launch duration) and
 the CPU baseline (the FP64 oracle port of the reference CpuSeq+KISS path timed
-on this host) -- plus the FP32-mixed rate and error on the 2048^2 form."""
+on this host's threads).  Further sections in the same line:
+* fp32_mixed: the FP32-mixed form batched (16 x 1024^2) and at 2048^2 (alone,
+  4 batched), against FP64 alone and batched, and its psi error after 1000
+  steps on both grids;
+* born_3d: 3D Born series, 128^3 x 32 energies (energy-iterations/s,
+  amplitudes against complex128 runs);
+* serving: the request loop's solves/s, p50/p99 latency and FP64 spot checks."""
 import argparse
 import ctypes
 import json
