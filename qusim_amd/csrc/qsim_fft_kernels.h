@@ -44,6 +44,7 @@ QSIM_P2CFG(double, 2048, 5, 16, 16, 8, 4, 4)
 QSIM_P2CFG4(double, 2048, 6, 8, 8, 8, 4, 1, 4)
 QSIM_P2CFG4(double, 2048, 7, 8, 8, 8, 4, 1, 2)
 QSIM_P2CFG(double, 4096, 0, 16, 16, 16, 0, 0)
+QSIM_P2CFG4(double, 4096, 1, 8, 8, 8, 8, 0, 0)  // 1D split jobs: 8 per thread
 // 16 elements per thread keeps these within 128 registers (32 per thread
 // spilled ~1 KiB per lane); radix 8 passes keep the padding within 160 KiB
 QSIM_P2CFG4(double, 8192, 0, 16, 8, 8, 8, 0, 0)  // 1D split jobs only
@@ -71,6 +72,7 @@ QSIM_P2CFG(float, 2048, 5, 16, 16, 8, 1, 4)
 QSIM_P2CFG4(float, 2048, 6, 8, 8, 8, 4, 1, 4)
 QSIM_P2CFG4(float, 2048, 7, 8, 8, 8, 4, 1, 2)
 QSIM_P2CFG(float, 4096, 0, 16, 16, 16, 0, 0)
+QSIM_P2CFG4(float, 4096, 1, 8, 8, 8, 8, 0, 0)  // 1D split jobs: 8 per thread
 QSIM_P2CFG4(float, 8192, 0, 16, 8, 8, 8, 0, 0)
 QSIM_P2CFG(float, 16384, 0, 16, 32, 32, 0, 0)  // 1D split jobs only (32 per thread: 512 threads, 256 registers)
 #undef QSIM_P2CFG

@@ -213,17 +213,30 @@ struct Prog1D {
     int k[3], e[3];
 };
 
+// Radix plan of the persistent 1D kernel per length.  A job is one workgroup
+// stepping its line on its own, so each step's chain of dependent work sets
+// the time: lengths 1024, 2048 and 4096 take four-pass plans with 8 elements
+// per thread (128 / 256 / 512 threads) instead of the 2D defaults (32 / 128 /
+// 256 threads).
 template <typename T, int N>
-__global__ void __launch_bounds__((P2Cfg<T, N>::Line::TPL))
+struct Split1DV {
+    static constexpr int V = (N == 1024 || N == 2048) ? 6 : (N == 4096 ? 1 : 0);
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__((P2Cfg<T, N, Split1DV<T, N>::V>::Line::TPL))
     k_split1d(const SplitDev *__restrict__ jobs, Prog1D prog, size_t nsteps,
               const c128 *__restrict__ tw)
 {
-    using P = typename P2Cfg<T, N>::Line;
+    using P = typename P2Cfg<T, N, Split1DV<T, N>::V>::Line;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const SplitDev &J = jobs[blockIdx.x];
     const int t = threadIdx.x;
     constexpr int RA = P::RA_, RZ = P::RLAST;
+    // with 8 elements per thread the step's factors fit in registers: their
+    // loads are issued at the top of the step and land during the forward FFT
+    constexpr bool PREFETCH = P::E <= 8;
     cx<T> v[P::E];
     c128 *psi = J.psi;
     const c128 *__restrict__ e0 = J.ev[prog.ev0];
@@ -241,27 +254,49 @@ __global__ void __launch_bounds__((P2Cfg<T, N>::Line::TPL))
             // pass out of the step loop into registers (that spilled)
             const c128 *twl = tw;
             __asm__ volatile("" : "+s"(twl));
-            P::forward(v, t, lds, twl);
-            const c128 *kt = J.kx[prog.k[ph]];
-            __asm__ volatile("" : "+s"(kt));  // likewise: reload each step
-#pragma unroll
-            for (int u = 0; u < P::E / RZ; ++u)
-#pragma unroll
-                for (int r = 0; r < RZ; ++r) {
-                    const int k = P::template in_pos<RZ>(t, u, r);
-                    v[u * RZ + r] = apply(v[u * RZ + r], ld_g(kt + k));
-                }
-            P::inverse(v, t, lds, twl);
             const bool last = (s + 1 == nsteps) && (ph + 1 == prog.nph);
-            const c128 *ev = J.ev[last ? prog.ev_end : prog.e[ph]];
-            __asm__ volatile("" : "+s"(ev));
+            if constexpr (PREFETCH) {
+                const c128 *kt = J.kx[prog.k[ph]];
+                const c128 *ev = J.ev[last ? prog.ev_end : prog.e[ph]];
+                c128 kp[P::E], ep[P::E];
 #pragma unroll
-            for (int u = 0; u < P::E / RA; ++u)
+                for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
-                for (int r = 0; r < RA; ++r) {
-                    const int pos = P::template in_pos<RA>(t, u, r);
-                    v[u * RA + r] = apply(v[u * RA + r], ld_g(ev + pos));
-                }
+                    for (int r = 0; r < RZ; ++r)
+                        kp[u * RZ + r] = ld_g(kt + P::template in_pos<RZ>(t, u, r));
+#pragma unroll
+                for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                    for (int r = 0; r < RA; ++r)
+                        ep[u * RA + r] = ld_g(ev + P::template in_pos<RA>(t, u, r));
+                P::forward(v, t, lds, twl);
+#pragma unroll
+                for (int i = 0; i < P::E; ++i) v[i] = apply(v[i], kp[i]);
+                P::inverse(v, t, lds, twl);
+#pragma unroll
+                for (int i = 0; i < P::E; ++i) v[i] = apply(v[i], ep[i]);
+            } else {
+                P::forward(v, t, lds, twl);
+                const c128 *kt = J.kx[prog.k[ph]];
+                __asm__ volatile("" : "+s"(kt));  // likewise: reload each step
+#pragma unroll
+                for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+                    for (int r = 0; r < RZ; ++r) {
+                        const int k = P::template in_pos<RZ>(t, u, r);
+                        v[u * RZ + r] = apply(v[u * RZ + r], ld_g(kt + k));
+                    }
+                P::inverse(v, t, lds, twl);
+                const c128 *ev = J.ev[last ? prog.ev_end : prog.e[ph]];
+                __asm__ volatile("" : "+s"(ev));
+#pragma unroll
+                for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                    for (int r = 0; r < RA; ++r) {
+                        const int pos = P::template in_pos<RA>(t, u, r);
+                        v[u * RA + r] = apply(v[u * RA + r], ld_g(ev + pos));
+                    }
+            }
         }
     }
 #pragma unroll
@@ -417,7 +452,7 @@ template <typename T, int N>
 int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps, const void *tw,
               hipStream_t s)
 {
-    using P = typename P2Cfg<T, N>::Line;
+    using P = typename P2Cfg<T, N, Split1DV<T, N>::V>::Line;
     auto fn = k_split1d<T, N>;
     const size_t lds = (size_t)P::LS * sizeof(cx<T>);
     QSIM_TRY(lds_limit((const void *)fn, lds));
