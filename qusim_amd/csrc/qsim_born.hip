@@ -116,12 +116,20 @@ __device__ __forceinline__ c128 source(c128 dpsi, c128 p0, c128 v, double eps)
     else return mk(v.x, v.y + eps) * dpsi + v.x * p0;                // op_Add3
 }
 
-// Launch variant of the Born sweeps per line length: 128-point lines run
-// 8 elements per thread (radices 8, 8, 2), which leaves registers for the
-// prefetched operands of the inner sweep; other lengths use variant 0.
+// Launch variant of the Born sweeps per line length: 8 elements per thread
+// from 128 to 4096 points.  At 128 (radices 8, 8, 2) that leaves registers for
+// the prefetched operands of the inner sweep.  From 256 up it also gives one
+// 2D solve (one energy, a few hundred lines) 4-16 times the waves of the
+// 16-32-element default plans.  BornLFV: the strided (line-fast) passes, where
+// 128-point lines keep their default plan (the 3D plane-sweep form).
 template <int N>
 struct BornV {
-    static constexpr int V = N == 128 ? 1 : 0;
+    static constexpr int V = (N == 1024 || N == 2048) ? 6
+                             : (N == 128 || N == 256 || N == 512 || N == 4096) ? 1 : 0;
+};
+template <int N>
+struct BornLFV {
+    static constexpr int V = N == 128 ? 0 : BornV<N>::V;
 };
 
 // Sum over the workgroup: wave reduction by lane shuffles, then one value
@@ -377,10 +385,10 @@ __global__ void __launch_bounds__(PLANE_THREADS)
 
 // Outer sweep: lines along axis 0, stride S = n1*n2, G0 between FFT and IFFT.
 template <typename T, int N>
-__global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
+__global__ void __launch_bounds__((P2Launch<T, N, true, BornLFV<N>::V>::THREADS))
     k_born_outer(const BornDev *__restrict__ jobs, BornGeom geo, const c128 *__restrict__ tw)
 {
-    using LP = P2Launch<T, N, true>;
+    using LP = P2Launch<T, N, true, BornLFV<N>::V>;
     using P = typename LP::Line;
     constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
@@ -430,10 +438,10 @@ __global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
 
 // Middle axis (3D y): plain transform of lines [n0][n1][n2] along axis 1.
 template <typename T, int N, bool INV>
-__global__ void __launch_bounds__((P2Launch<T, N, true>::THREADS))
+__global__ void __launch_bounds__((P2Launch<T, N, true, BornLFV<N>::V>::THREADS))
     k_born_mid(const BornDev *__restrict__ jobs, BornGeom geo, const c128 *__restrict__ tw)
 {
-    using LP = P2Launch<T, N, true>;
+    using LP = P2Launch<T, N, true, BornLFV<N>::V>;
     using P = typename LP::Line;
     constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
@@ -679,7 +687,7 @@ template <typename T, int N>
 int launch_outer_t(const BornDev *tab, int nact, const BornGeom &geo, const void *tw,
                    hipStream_t s)
 {
-    using LP = P2Launch<T, N, true>;
+    using LP = P2Launch<T, N, true, BornLFV<N>::V>;
     auto fn = k_born_outer<T, N>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t S = geo.n1 * geo.n2;
@@ -701,7 +709,7 @@ template <typename T, int N, bool INV>
 int launch_mid_t(const BornDev *tab, int nact, const BornGeom &geo, const void *tw,
                  hipStream_t s)
 {
-    using LP = P2Launch<T, N, true>;
+    using LP = P2Launch<T, N, true, BornLFV<N>::V>;
     auto fn = k_born_mid<T, N, INV>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t nlines = geo.n0 * geo.n2;

@@ -25,7 +25,9 @@ QSIM_P2CFG(double, 64, 0, 8, 8, 1, 0, 0)
 QSIM_P2CFG(double, 128, 0, 16, 8, 1, 0, 0)
 QSIM_P2CFG(double, 128, 1, 8, 8, 2, 0, 0)  // 8 per thread (Born sweeps)
 QSIM_P2CFG(double, 256, 0, 16, 16, 1, 0, 0)
+QSIM_P2CFG(double, 256, 1, 8, 8, 4, 0, 0)  // 8 per thread (Born sweeps)
 QSIM_P2CFG(double, 512, 0, 8, 8, 8, 0, 0)
+QSIM_P2CFG(double, 512, 1, 8, 8, 8, 0, 0)  // Born sweeps (as variant 0)
 QSIM_P2CFG(double, 1024, 0, 16, 16, 4, 1, 4)
 QSIM_P2CFG(double, 1024, 1, 16, 16, 4, 4, 4)
 QSIM_P2CFG(double, 1024, 2, 32, 32, 1, 2, 4)
@@ -54,7 +56,9 @@ QSIM_P2CFG(float, 64, 0, 8, 8, 1, 0, 0)
 QSIM_P2CFG(float, 128, 0, 16, 8, 1, 0, 0)
 QSIM_P2CFG(float, 128, 1, 8, 8, 2, 0, 0)  // 8 per thread (Born sweeps)
 QSIM_P2CFG(float, 256, 0, 16, 16, 1, 0, 0)
+QSIM_P2CFG(float, 256, 1, 8, 8, 4, 0, 0)  // 8 per thread (Born sweeps)
 QSIM_P2CFG(float, 512, 0, 16, 32, 1, 0, 0)
+QSIM_P2CFG(float, 512, 1, 8, 8, 8, 0, 0)  // 8 per thread (Born sweeps)
 QSIM_P2CFG(float, 1024, 0, 32, 32, 1, 2, 8)
 QSIM_P2CFG(float, 1024, 1, 32, 32, 1, 4, 16)
 QSIM_P2CFG(float, 1024, 2, 16, 16, 4, 1, 8)

@@ -171,3 +171,46 @@ def test_c4_grid_fp64_matches_oracle(oracle):
     err = rel_l2(s.GetPsi(), ref)
     print("C4 grid FP64 x3 rel L2 %.2e" % err)
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(256, 256), (512, 512), (1024, 1024), (2048, 256)])
+def test_born2d_large_lines_match_oracle(shape, oracle):
+    """2D solves whose lines run the 8-element plans (256 to 2048 points), in
+    complex128 against the oracle and in FP32 against complex128"""
+    from qusim_amd.api import QuPerturbation2D, SolverMethod
+    # dx = 0.5: every side at least 24 wavelengths, the preconditioned
+    # series' size rule (src/Perturbation2DImpl.cpp:25-31)
+    lo = tuple(-s / 4.0 for s in shape)
+    hi = tuple(s / 4.0 for s in shape)
+    V = pot2d(shape, lo, hi)
+    E, d = 2.0, (1.0, 0.3)
+    got = {}
+    for prec in ("fp64", "mixed32"):
+        s = QuPerturbation2D().init(V, lo[0], hi[0], shape[0], lo[1], hi[1], shape[1], E, 0.0,
+                                    d[0], d[1], SolverMethod.BornSerise, 1.0, 1.0,
+                                    opts(1, 1, 0.5, 6, precision=prec))
+        s.Compute()
+        got[prec] = s.GetPsi()
+    ref, _, _, _ = oracle.born(V, lo, hi, E, d, order=6, preconditional=1, preconditioner=1,
+                               slow=0.5, threads=8)
+    assert rel_l2(got["fp64"], ref) < 1e-12
+    assert rel_l2(got["mixed32"], got["fp64"]) < 1e-5
+
+
+def test_born3d_long_axes_match_oracle(oracle):
+    """3D with 256- and 512-point axes (inner, middle and outer passes on the
+    8-element plans)"""
+    from qusim_amd.api import QuPerturbation3D, SolverMethod
+    shape = (512, 16, 256)
+    # dx = 0.75: every side at least 20/k, the 3D size rule
+    lo = tuple(-0.375 * s for s in shape)
+    hi = tuple(0.375 * s for s in shape)
+    V = pot3d(shape, lo, hi)
+    E, d = 3.0, (0.2, 0.0, 1.0)
+    s = QuPerturbation3D().init(V, lo[0], hi[0], shape[0], lo[1], hi[1], shape[1], lo[2], hi[2],
+                                shape[2], E, 0.0, d[0], d[1], d[2], SolverMethod.BornSerise,
+                                1.0, 1.0, opts(1, 1, 0.5, 3))
+    s.Compute()
+    ref, _, _, _ = oracle.born(V, lo, hi, E, d, order=3, preconditional=1, preconditioner=1,
+                               slow=0.5, threads=8)
+    assert rel_l2(s.GetPsi(), ref) < 1e-12
