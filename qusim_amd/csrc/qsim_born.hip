@@ -116,16 +116,15 @@ __device__ __forceinline__ c128 source(c128 dpsi, c128 p0, c128 v, double eps)
     else return mk(v.x, v.y + eps) * dpsi + v.x * p0;                // op_Add3
 }
 
-// Launch variant of the Born sweeps per line length: 8 elements per thread
-// from 128 to 4096 points.  At 128 (radices 8, 8, 2) that leaves registers for
-// the prefetched operands of the inner sweep.  From 256 up it also gives one
-// 2D solve (one energy, a few hundred lines) 4-16 times the waves of the
-// 16-32-element default plans.  BornLFV: the strided (line-fast) passes, where
+// Launch variant of the Born sweeps per line length (LineV: 8 elements per
+// thread from 128 to 4096 points).  At 128 (radices 8, 8, 2) that leaves
+// registers for the prefetched operands of the inner sweep; from 256 up it
+// gives one 2D solve (one energy, a few hundred lines) 4-16 times the waves
+// of the default plans.  BornLFV: the strided (line-fast) passes, where
 // 128-point lines keep their default plan (the 3D plane-sweep form).
 template <int N>
 struct BornV {
-    static constexpr int V = (N == 1024 || N == 2048) ? 6
-                             : (N == 128 || N == 256 || N == 512 || N == 4096) ? 1 : 0;
+    static constexpr int V = LineV<N>::V;
 };
 template <int N>
 struct BornLFV {

@@ -57,7 +57,7 @@ template <typename T, int N, bool LF, bool INV>
 static int launch_p2_inst(const void *in, void *out, size_t S, size_t nlines, const void *tw,
                           double scale, hipStream_t s)
 {
-    using LP = P2Launch<T, N, LF>;
+    using LP = P2Launch<T, N, LF, FftV<T, N>::V>;
     auto fn = k_fft_p2<T, N, LF, INV>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t blocks = (nlines + LP::L - 1) / LP::L;
@@ -137,7 +137,7 @@ static int launch_4s_inst(const void *in, void *out, size_t S, size_t nlines, si
                           const void *tw, const void *twn, size_t nbig, double scale,
                           hipStream_t s)
 {
-    using LP = P2Launch<T, N, LF>;
+    using LP = P2Launch<T, N, LF, FftV<T, N>::V>;
     auto fn = k_fft_4step<T, N, LF, INV, PASS>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t blocks = (nlines + LP::L - 1) / LP::L;

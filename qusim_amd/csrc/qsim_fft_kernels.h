@@ -111,13 +111,31 @@ struct P2Launch {
     static constexpr size_t LDS_BYTES = (size_t)L * LSTR * sizeof(cx<T>);
 };
 
+// Radix plan of the Born sweeps per length: 8 elements per thread from 128
+// to 4096 points, so a solve of one energy on a few hundred lines still puts
+// several waves on every SIMD.
+template <int N>
+struct LineV {
+    static constexpr int V = (N == 1024 || N == 2048) ? 6
+                             : (N == 128 || N == 256 || N == 512 || N == 4096) ? 1 : 0;
+};
+
+// Radix plan of the FFT plan kernels: the default plans, except FP32 512 and
+// 1024, whose 16-32-element plans leave a 2D transform of that size a
+// fraction of a wave per SIMD (measured: 512^2 15.0 -> 10.5 us, 1024^2 23.0
+// -> 20.4 us; at 2048 and for FP64 the default plans are faster).
+template <typename T, int N>
+struct FftV {
+    static constexpr int V = sizeof(T) == 4 && (N == 512 || N == 1024) ? LineV<N>::V : 0;
+};
+
 // Plain transform of `nlines` lines of length N with stride S (in may be out).
 template <typename T, int N, bool LF, bool INV>
-__global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
+__global__ void __launch_bounds__((P2Launch<T, N, LF, FftV<T, N>::V>::THREADS))
     k_fft_p2(const cx<T> *in, cx<T> *out, size_t S, size_t nlines,
              const c128 *__restrict__ tw, T scale)
 {
-    using LP = P2Launch<T, N, LF>;
+    using LP = P2Launch<T, N, LF, FftV<T, N>::V>;
     using P = typename LP::Line;
     constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
@@ -159,12 +177,12 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
 //         k2 + n2*k1 (stride n2*S), times `scale`.
 // `other` is n1 for PASS 0 and n2 for PASS 1; twn is the table of length n.
 template <typename T, int N, bool LF, bool INV, int PASS>
-__global__ void __launch_bounds__((P2Launch<T, N, LF>::THREADS))
+__global__ void __launch_bounds__((P2Launch<T, N, LF, FftV<T, N>::V>::THREADS))
     k_fft_4step(const cx<T> *in, cx<T> *out, size_t S, size_t nlines, size_t other,
                 const c128 *__restrict__ tw, const c128 *__restrict__ twn, size_t nbig,
                 T scale)
 {
-    using LP = P2Launch<T, N, LF>;
+    using LP = P2Launch<T, N, LF, FftV<T, N>::V>;
     using P = typename LP::Line;
     constexpr int L = LP::L;
     extern __shared__ unsigned char smem_raw[];
