@@ -514,6 +514,8 @@ struct qsim_born_job {
     BornDev *dev = nullptr;     // device copy of `host`
     BornDev *active_tab = nullptr;
     BornDev *tab_host = nullptr;  // pinned staging of the active table
+    double *rtot_dev = nullptr;   // per active energy: residual and norm totals
+    double *rtot_host = nullptr;  // pinned copy of rtot_dev
     std::vector<void *> owned;
     std::vector<bool> started;  // tmp holds a started iteration
     std::vector<int> iters;
@@ -732,6 +734,7 @@ void destroy(qsim_born_job *j)
     if (!j) return;
     for (void *p : j->owned) (void)hipFree(p);
     if (j->tab_host) (void)hipHostFree(j->tab_host);
+    if (j->rtot_host) (void)hipHostFree(j->rtot_host);
     if (j->fwd) qsim_hip_fft_plan_destroy(j->fwd);
     if (j->inv) qsim_hip_fft_plan_destroy(j->inv);
     delete j;
@@ -883,6 +886,10 @@ int create(qsim_born_job *j)
     j->active_tab = (BornDev *)p;
     QSIM_HIP(hipHostMalloc(&p, sizeof(BornDev) * j->nE, hipHostMallocDefault));
     j->tab_host = (BornDev *)p;
+    QSIM_TRY(balloc(j, &p, 2 * sizeof(double) * j->nE));
+    j->rtot_dev = (double *)p;
+    QSIM_HIP(hipHostMalloc(&p, 2 * sizeof(double) * j->nE, hipHostMallocDefault));
+    j->rtot_host = (double *)p;
     if (j->fused) {
         QSIM_TRY(twiddle_table(g.n0, j->prec, &j->tw[0]));
         QSIM_TRY(twiddle_table(g.n1, j->prec, &j->tw[1]));
@@ -920,21 +927,40 @@ int fused_pass_set(qsim_born_job *j, int mode, const BornDev *tab, int nact, hip
     return QSIM_OK;
 }
 
+// Totals of the per-block residual sums of each active energy (tab order),
+// one thread per energy adding the blocks in block order.
+__global__ void k_resid_total(const BornDev *__restrict__ tab, int nact, size_t nb,
+                              double *__restrict__ out)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nact) return;
+    const double *p = tab[a].resid;
+    double r = 0, nn = 0;
+    for (size_t b = 0; b < nb; ++b) {
+        r += p[2 * b];
+        nn += p[2 * b + 1];
+    }
+    out[2 * a] = r;
+    out[2 * a + 1] = nn;
+}
+
+// Relative update of each active energy, sqrt(sum |d new - d old|^2 /
+// sum |d new|^2); `act` lists the energies of the device's active table in
+// its order.  One small kernel and one copy, then a wait on the stream.
 int read_residuals(qsim_born_job *j, const std::vector<int> &act, hipStream_t s,
                    std::vector<double> &out)
 {
-    const size_t nb = j->inner_blocks;
-    std::vector<double> h(2 * nb);
+    const int nact = (int)act.size();
     out.assign(act.size(), 0.0);
+    if (nact == 0) return QSIM_OK;
+    hipLaunchKernelGGL(k_resid_total, dim3((nact + 63) / 64), dim3(64), 0, s,
+                       (const BornDev *)j->active_tab, nact, j->inner_blocks, j->rtot_dev);
+    QSIM_HIP(hipGetLastError());
+    QSIM_HIP(hipMemcpyAsync(j->rtot_host, j->rtot_dev, 2 * sizeof(double) * nact,
+                            hipMemcpyDeviceToHost, s));
     QSIM_HIP(hipStreamSynchronize(s));
-    for (size_t a = 0; a < act.size(); ++a) {
-        QSIM_HIP(hipMemcpy(h.data(), j->host[act[a]].resid, 2 * nb * sizeof(double),
-                           hipMemcpyDeviceToHost));
-        double r = 0, nn = 0;
-        for (size_t b = 0; b < nb; ++b) {
-            r += h[2 * b];
-            nn += h[2 * b + 1];
-        }
+    for (int a = 0; a < nact; ++a) {
+        const double r = j->rtot_host[2 * a], nn = j->rtot_host[2 * a + 1];
         out[a] = nn > 0 ? std::sqrt(r / nn) : 0.0;
     }
     return QSIM_OK;
