@@ -13,6 +13,7 @@ import math
 import numpy as np
 
 from . import _native as N
+from . import snapshot
 from .fft import FourierTransform
 
 
@@ -253,6 +254,35 @@ class Evolver:
     def GetMethod(self):
         return self._method
 
+    def _snapshot_header(self):
+        d = len(self._shape)
+        return {"dim": d, "extent": list(self._extent), "dt": [self._dt.real, self._dt.imag],
+                "mass": self._mass, "hbar": self._hbar,
+                "method": "O2" if self._method == SolverMethod.SplittingMethodO2 else "O4",
+                "boundary": self._boundary.name}
+
+    def SaveSnapshot(self, path):
+        """Write psi and the step count to `path` (format in
+        qusim_amd/snapshot.py).  No counterpart in the reference."""
+        h = self._snapshot_header()
+        h.update(step=self.fStep, precision=self._prec)
+        snapshot.write(path, h, self.psi.download().reshape(self._shape))
+
+    def LoadSnapshot(self, path):
+        """Replace psi and the step count with a snapshot's.  The grid, dt,
+        mass, hbar, method and boundary must equal this evolver's (ValueError
+        otherwise); the precision may differ, since psi is complex128 either
+        way.  The next steps then continue the saved evolution bit for bit."""
+        h, psi = snapshot.read(path)
+        mine = self._snapshot_header()
+        if tuple(psi.shape) != tuple(self._shape):
+            raise ValueError("snapshot grid %s, evolver grid %s" % (psi.shape, self._shape))
+        for k, v in mine.items():
+            if h.get(k) != v:
+                raise ValueError("snapshot %s %r, evolver %s %r" % (k, h.get(k), k, v))
+        self.psi.upload(psi)
+        self.fStep = int(h["step"])
+
     def close(self):
         if getattr(self, "_job", None):
             N.lib().qsim_hip_split_destroy(self._job)
@@ -275,6 +305,8 @@ class Evolver1D(Evolver):
         xs = self.fDx * np.arange(self.fNx) + self.fX0
         psi_h = psi if isinstance(psi, np.ndarray) else _eval1(psi, xs)
         V = v if isinstance(v, np.ndarray) else _eval1(v, xs)
+        self._shape = (self.fNx,)
+        self._extent = (self.fX0, self.fX1)
         self._setup(psi_h, np.asarray(V, np.complex128), force_normalization, dt,
                     force_normalization_each_step, b, solver, mass, hbar, self.fDx, opts, 1,
                     self.fNx, 1, self.fX0, self.fX1, 0.0, 1.0)
@@ -339,6 +371,8 @@ class Evolver2D(Evolver):
         X, Y = np.meshgrid(xs, ys, indexing="ij")
         psi_h = psi if isinstance(psi, np.ndarray) else _eval2(psi, X, Y)
         V = v if isinstance(v, np.ndarray) else _eval2(v, X, Y)
+        self._shape = (self.fNx, self.fNy)
+        self._extent = (self.fX0, float(x1), self.fY0, float(y1))
         self._setup(psi_h, np.asarray(V, np.complex128), force_normalization, dt,
                     force_normalization_each_step, b, solver, mass, hbar, self.fDx * self.fDy,
                     opts, 2, self.fNx, self.fNy, self.fX0, float(x1), self.fY0, float(y1))

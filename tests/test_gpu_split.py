@@ -240,3 +240,58 @@ def test_launch_variants_match_oracle(shape, variant, oracle):
     ref = oracle.split2d(psi, V, L0[0], L0[1], L1[0], L1[1], 0.05, 6, method=2, complex_v=True)
     assert rel_l2(outs["fp64"], ref) < 1e-12
     assert rel_l2(outs["mixed32"], outs["fp64"]) < 1e-5
+
+
+def _evolver(dim, precision, method, dt=0.05):
+    from qusim_amd.api import BoundaryCondition, Evolver1D, Evolver2D, Options, SolverMethod
+    o = Options().Hip().ComplexPotential(True)
+    if precision != "fp64":
+        o.Precision(precision)
+    sm = SolverMethod.SplittingMethodO2 if method == 2 else SolverMethod.SplittingMethodO4
+    if dim == 2:
+        psi, V, L0, L1 = gauss2d(128, 64)
+        return Evolver2D().init(psi, True, dt, False, V, L0[0], L0[1], 128, L1[0], L1[1], 64,
+                                BoundaryCondition.Period, sm, 1.0, 1.0, o)
+    x = -40 + 80.0 / 1024 * np.arange(1024)
+    psi = np.exp(1j * x - (x + 10) ** 2 / 16)
+    V = 0.6 * np.exp(-x ** 2 / 2) - 0.02j * np.exp(-(x - 5) ** 2 / 50)
+    return Evolver1D().init(psi, True, dt, False, V, -40.0, 40.0, 1024,
+                            BoundaryCondition.Period, sm, 1.0, 1.0, o)
+
+
+@pytest.mark.parametrize("dim,precision,method",
+                         [(2, "fp64", 2), (2, "mixed32", 2), (2, "fp64", 4), (1, "fp64", 2),
+                          (1, "mixed32", 4)])
+def test_snapshot_resume_is_bitwise(tmp_path, dim, precision, method):
+    """10 steps, SaveSnapshot, LoadSnapshot into a fresh evolver, 15 steps:
+    psi and Time() equal one evolver's 10 + 15 steps bit for bit."""
+    path = str(tmp_path / "psi.qpsi")
+    a = _evolver(dim, precision, method)
+    a.steps(10)
+    a.SaveSnapshot(path)
+    a.steps(15)
+    b = _evolver(dim, precision, method)
+    b.LoadSnapshot(path)
+    assert b.fStep == 10
+    b.steps(15)
+    assert b.Time() == a.Time()
+    assert np.array_equal(b.GetPsi().view(np.uint64), a.GetPsi().view(np.uint64))
+    a.close()
+    b.close()
+
+
+def test_snapshot_of_another_setup_is_refused(tmp_path):
+    path = str(tmp_path / "psi.qpsi")
+    a = _evolver(2, "fp64", 2)
+    a.steps(2)
+    a.SaveSnapshot(path)
+    with pytest.raises(ValueError, match="dt"):
+        _evolver(2, "fp64", 2, dt=0.1).LoadSnapshot(path)
+    with pytest.raises(ValueError, match="method"):
+        _evolver(2, "fp64", 4).LoadSnapshot(path)
+    with pytest.raises(ValueError, match="grid"):
+        _evolver(1, "fp64", 2).LoadSnapshot(path)
+    # the precision may differ: psi is complex128 in both forms
+    c = _evolver(2, "mixed32", 2)
+    c.LoadSnapshot(path)
+    assert np.array_equal(c.GetPsi(), a.GetPsi())
