@@ -22,6 +22,10 @@ int hip_error(hipError_t e, const char *what)
 {
     std::string m = std::string("HIP error ") + hipGetErrorName(e) + " (" +
                     hipGetErrorString(e) + ") in " + what;
+    // the runtime also keeps the error as the thread's last error; clear it,
+    // or the launch check (hipGetLastError) after the next kernel would
+    // report it again as that launch's failure
+    (void)hipGetLastError();
     return set_error(e == hipErrorOutOfMemory ? QSIM_ERR_NOMEM : QSIM_ERR_HIP, m);
 }
 
@@ -165,6 +169,7 @@ int qsim_hip_malloc(void **ptr, size_t bytes)
     hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) {
         *ptr = nullptr;
+        (void)hipGetLastError();  // not the next launch's error (see hip_error)
         return set_error(QSIM_ERR_NOMEM, "can't alloc " + std::to_string(bytes) +
                                              " bytes on the device: " + hipGetErrorString(e));
     }

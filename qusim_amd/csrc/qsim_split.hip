@@ -1191,8 +1191,10 @@ int qsim_hip_split_batch_create(qsim_split_batch **b, int capacity)
     if (hipMalloc(&p->table, sizeof(SplitDev) * capacity) != hipSuccess ||
         hipHostMalloc(&p->staged, sizeof(SplitDev) * capacity, hipHostMallocDefault) !=
             hipSuccess ||
-        hipEventCreateWithFlags(&p->copied, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&p->copied, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();  // not the next launch's error
         rc = set_error(QSIM_ERR_NOMEM, "batch: allocation failed");
+    }
     if (rc != QSIM_OK) {
         qsim_hip_split_batch_destroy(p);
         return rc;

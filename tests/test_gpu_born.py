@@ -214,3 +214,28 @@ def test_born3d_long_axes_match_oracle(oracle):
     ref, _, _, _ = oracle.born(V, lo, hi, E, d, order=3, preconditional=1, preconditioner=1,
                                slow=0.5, threads=8)
     assert rel_l2(s.GetPsi(), ref) < 1e-12
+
+
+def test_too_small_box_is_refused_and_the_library_stays_usable():
+    """Preconditioned solves need a box of at least 20/k0 per axis in 3D and
+    24 wavelengths in 2D (src/Perturbation3DImpl.cpp:34-51,
+    src/Perturbation2DImpl.cpp:25-31); the refused create leaves nothing
+    behind and the next solve runs.  Energies must be positive."""
+    from qusim_amd.api import Options, QuPerturbation2D, QuPerturbation3D, SolverMethod
+    pc = Options().Hip().Preconditional(True).VellekoopPreconditioner()
+    V3 = np.zeros((32, 32, 32), np.complex128)
+    with pytest.raises(RuntimeError, match="too small size"):
+        QuPerturbation3D().init(V3, -2, 2, 32, -2, 2, 32, -2, 2, 32, 0.5, 0.0, 0, 0, 1,
+                                SolverMethod.BornSerise, 1.0, 1.0, pc)
+    V2 = np.full((64, 64), 0.01, np.complex128)
+    with pytest.raises(RuntimeError, match="too small size"):
+        QuPerturbation2D().init(V2, -2, 2, 64, -2, 2, 64, 0.5, 0.0, 1, 0,
+                                SolverMethod.BornSerise, 1.0, 1.0, pc)
+    with pytest.raises(RuntimeError, match="energy must be positive"):
+        QuPerturbation2D().init(V2, -2, 2, 64, -2, 2, 64, -0.5, 0.0, 1, 0,
+                                SolverMethod.BornSerise, 1.0, 1.0, Options().Hip())
+    # the rule belongs to the preconditioned series: the plain one accepts the box
+    s = QuPerturbation2D().init(V2, -2, 2, 64, -2, 2, 64, 0.5, 0.0, 1, 0,
+                                SolverMethod.BornSerise, 1.0, 1.0, Options().Hip().Order(3))
+    s.Compute()
+    assert np.all(np.isfinite(s.GetPsi())) and np.abs(s.GetPsi()).max() > 0

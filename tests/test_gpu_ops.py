@@ -135,3 +135,13 @@ def test_xsection3d_reduction(N):
     ph = (x0 + I * dx) * kx + (y0 + J * dy) * ky + (z0 + K * dz) * kz
     want = np.sum((p0 + p) * V.real * np.exp(-1j * ph.ravel()))
     assert abs(got - want) / abs(want) < 1e-13
+
+
+def test_failed_allocation_raises_and_is_not_reported_again(N):
+    """An allocation beyond HBM raises (src/DeviceTemplate.h:327-334 throws
+    too), and the runtime's stored copy of that error does not reappear as
+    the failure of the next kernel launch."""
+    with pytest.raises(RuntimeError, match="can't alloc"):
+        N.DeviceArray((1 << 41,), np.uint8)   # 2 TiB
+    a = up(N, np.ones(1000, np.complex128))
+    assert N.reduce1("qsim_hip_norm2", a.ptr, 1000) == 1000.0

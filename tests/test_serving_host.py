@@ -112,3 +112,21 @@ def test_dead_born_worker_stops_the_run(monkeypatch):
     reqs = [serving.Request(i, "born2d", 256, 200, 0.0, {}) for i in range(2)]
     with pytest.raises(RuntimeError, match="Born worker stopped"):
         serving.Server(precision=32, realtime=False, born_workers=1).run(reqs)
+
+
+def test_request_beyond_the_budget_is_refused_and_the_rest_complete(monkeypatch):
+    """a request whose footprint exceeds the whole HBM budget is refused
+    with an error at admission (it would otherwise wait forever); the
+    others are admitted and finish"""
+    serving = _host_only(monkeypatch)
+    reqs = [serving.Request(0, "born2d", 256, 200, 0.0, {}),
+            serving.Request(1, "born2d", 1024, 200, 0.0, {}),
+            serving.Request(2, "born2d", 256, 200, 0.0, {})]
+    budget = serving.footprint(reqs[0], 32) * 2
+    assert serving.footprint(reqs[1], 32) > budget
+    s = serving.Server(precision=32, hbm_budget=budget, realtime=False, born_workers=2)
+    st = s.run(reqs)
+    assert set(s.errors) == {1} and "exceeds the HBM budget" in s.errors[1]
+    assert set(s.results) == {0, 2}
+    assert st["completed"] == 2 and st["errors"] == 1
+    assert s.peak_used <= budget
