@@ -24,6 +24,7 @@ HIP_SOURCES = [
 HOST_SOURCES = ["hip_device.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 ARCH = os.environ.get("QSIM_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
@@ -64,7 +65,8 @@ def build_hip(jobs=8):
         list(ex.map(_run, cmds))
     out = os.path.join(HERE, "libqsim_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+         + ["-L" + ROCM_LIB, "-Wl,-rpath," + ROCM_LIB, "-lrocprofiler-sdk-roctx"])
     os.replace(tmp, out)
     return out
 

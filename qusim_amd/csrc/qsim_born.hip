@@ -1140,6 +1140,7 @@ int qsim_hip_born_create(qsim_born_job **job, const qsim_born_desc *d, const qsi
 int qsim_hip_born_iterate(qsim_born_job *j, int max_iter, double tol, int *iters,
                           double *residual, qsim_stream_t stream)
 {
+    qsim::TraceRange tr_("qsim_hip_born_iterate");
     QSIM_CHECK(j, "null job");
     QSIM_CHECK(max_iter >= 0, "max_iter must be >= 0");
     hipStream_t s = as_stream(stream);

@@ -175,6 +175,20 @@ int device_cu_count();
 // (kernel, device) for the whole process and under a lock: the attribute is
 // process state, and request loops launch from several threads at once.
 int lds_limit(const void *kernel, size_t bytes);
+// A roctx range (rocprofv3 --marker-trace) around an entry point, recorded
+// only when the environment sets QSIM_ROCTX=1; otherwise it costs a branch.
+// Asynchronous entry points mark the host-side enqueue, synchronous ones the
+// whole call.
+class TraceRange {
+public:
+    explicit TraceRange(const char *name);
+    ~TraceRange();
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+
+private:
+    bool on_;
+};
 }  // namespace qsim
 
 #define QSIM_HIP(call)                                                \

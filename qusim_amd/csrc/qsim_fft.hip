@@ -315,6 +315,7 @@ int qsim_hip_fft_plan_create(qsim_fft_plan **plan, int rank, const size_t *dims,
 
 int qsim_hip_fft_exec(qsim_fft_plan *p, const void *in, void *out, qsim_stream_t s)
 {
+    qsim::TraceRange tr_("qsim_hip_fft_exec");
     QSIM_CHECK(p && in && out, "null argument to qsim_hip_fft_exec");
     hipStream_t st = as_stream(s);
     // inner axis first, as the reference transforms (src/FourierTransform.cpp:248-284)

@@ -1,10 +1,13 @@
 // qsim_runtime.hip -- error state, device selection, memory, streams, events
 // and the per-device reduction scratch of libqsim_hip.so.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <set>
 #include <utility>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "qsim_common.h"
 
@@ -16,6 +19,25 @@ int set_error(int code, const std::string &msg)
 {
     g_last_error = msg;
     return code;
+}
+
+static bool trace_enabled()
+{
+    static const bool on = [] {
+        const char *v = std::getenv("QSIM_ROCTX");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
+
+TraceRange::TraceRange(const char *name) : on_(trace_enabled())
+{
+    if (on_) roctxRangePushA(name);
+}
+
+TraceRange::~TraceRange()
+{
+    if (on_) roctxRangePop();
 }
 
 int hip_error(hipError_t e, const char *what)

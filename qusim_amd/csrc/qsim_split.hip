@@ -898,6 +898,7 @@ int qsim_hip_split_create(qsim_split_job **job, const qsim_split_desc *d, const 
 
 int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
 {
+    qsim::TraceRange tr_("qsim_hip_split_steps");
     QSIM_CHECK(j, "null job");
     if (nsteps == 0) return QSIM_OK;
     hipStream_t s = as_stream(stream);
@@ -1079,6 +1080,7 @@ extern "C" {
 int qsim_hip_split_autotune(qsim_split_job *j, int reps, int *row_variant, int *col_variant,
                             double *row_us, double *col_us, qsim_stream_t stream)
 {
+    qsim::TraceRange tr_("qsim_hip_split_autotune");
     QSIM_CHECK(j, "null job");
     QSIM_CHECK(reps >= 1, "reps must be >= 1");
     if (!j->fused || j->d.dim != 2) {
@@ -1094,6 +1096,7 @@ int qsim_hip_split_autotune_batched(qsim_split_job *const *jobs, int njobs, int 
                                     int *row_variant, int *col_variant, double *row_us,
                                     double *col_us, qsim_stream_t stream)
 {
+    qsim::TraceRange tr_("qsim_hip_split_autotune_batched");
     QSIM_CHECK(jobs && njobs >= 1, "null job list");
     QSIM_CHECK(reps >= 1, "reps must be >= 1");
     qsim_split_job *j0 = jobs[0];
@@ -1136,6 +1139,7 @@ int qsim_hip_split_destroy(qsim_split_job *j)
 int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps,
                                  qsim_stream_t stream)
 {
+    qsim::TraceRange tr_("qsim_hip_split_steps_batched");
     QSIM_CHECK(jobs && njobs >= 0, "null job list");
     if (njobs == 0 || nsteps == 0) return QSIM_OK;
     qsim_split_job *j0 = jobs[0];
@@ -1206,6 +1210,7 @@ int qsim_hip_split_batch_create(qsim_split_batch **b, int capacity)
 int qsim_hip_split_batch_steps(qsim_split_batch *b, qsim_split_job *const *jobs, int njobs,
                                size_t nsteps, qsim_stream_t stream)
 {
+    qsim::TraceRange tr_("qsim_hip_split_batch_steps");
     QSIM_CHECK(b && jobs && njobs >= 0, "batch: null argument");
     QSIM_CHECK(njobs <= b->cap, "batch: more jobs than the batch's capacity");
     if (njobs == 0 || nsteps == 0) return QSIM_OK;

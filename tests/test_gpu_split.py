@@ -295,3 +295,24 @@ def test_snapshot_of_another_setup_is_refused(tmp_path):
     c = _evolver(2, "mixed32", 2)
     c.LoadSnapshot(path)
     assert np.array_equal(c.GetPsi(), a.GetPsi())
+
+
+def test_roctx_ranges_on_leave_results_unchanged():
+    """With QSIM_ROCTX=1 the entry points push and pop roctx ranges (no
+    profiler attached here): the same steps give the same bits."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    prog = (
+        "import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "from test_gpu_split import _evolver\n"
+        "e = _evolver(2, 'fp64', 2); e.steps(7)\n"
+        "sys.stdout.buffer.write(e.GetPsi().tobytes())\n" % (root, os.path.join(root, "tests")))
+    outs = []
+    for flag in ("0", "1"):
+        r = subprocess.run([sys.executable, "-c", prog], capture_output=True, timeout=120,
+                           env=dict(os.environ, QSIM_ROCTX=flag))
+        assert r.returncode == 0, r.stderr.decode()[-2000:]
+        outs.append(r.stdout)
+    assert len(outs[0]) == 128 * 64 * 16 and outs[0] == outs[1]
