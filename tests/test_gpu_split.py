@@ -316,3 +316,35 @@ def test_roctx_ranges_on_leave_results_unchanged():
         assert r.returncode == 0, r.stderr.decode()[-2000:]
         outs.append(r.stdout)
     assert len(outs[0]) == 128 * 64 * 16 and outs[0] == outs[1]
+
+
+def test_c2_full_run_matches_oracle(oracle):
+    """C2 as configured: 1024^2 on [-150,150)^2, complex V with the absorbing
+    layer, dt = 0.2, all 500 steps, against the FP64 oracle (16 threads)."""
+    from qusim_amd.workloads import c2_problem
+    p = c2_problem()
+    assert p["steps"] == 500 and p["psi"].shape == (1024, 1024)
+    L = (p["x0"], p["x1"])
+    got = run_2d(p["psi"], p["V"], L, L, p["dt"], p["steps"], 2, complex_v=True).GetPsi()
+    ref = oracle.split2d(p["psi"], p["V"], L[0], L[1], L[0], L[1], p["dt"], p["steps"],
+                         method=2, complex_v=True, threads=16)
+    err = rel_l2(got, ref)
+    print("C2 1024^2 x500 rel L2 vs oracle", err)
+    assert err < 1e-12
+
+
+def test_c3_full_run_fp32_within_1e5_of_fp64():
+    """C3 as configured: 2048^2, dt = 0.1, 1000 steps; the FP32-mixed psi
+    against the complex128 device path (pinned to the oracle above) stays
+    within 1e-5 relative L2, and the norms agree to 1e-5."""
+    from qusim_amd.workloads import c3_problem
+    p = c3_problem()
+    assert p["steps"] == 1000 and p["psi"].shape == (2048, 2048)
+    L = (p["x0"], p["x1"])
+    a = run_2d(p["psi"], p["V"], L, L, p["dt"], p["steps"], 2, complex_v=True).GetPsi()
+    b = run_2d(p["psi"], p["V"], L, L, p["dt"], p["steps"], 2, precision="mixed32",
+               complex_v=True).GetPsi()
+    err = rel_l2(b, a)
+    na, nb = np.vdot(a, a).real, np.vdot(b, b).real
+    print("C3 2048^2 x1000 FP32-mixed rel L2 vs complex128", err, "norm drift", nb / na - 1)
+    assert err < 1e-5 and abs(nb / na - 1) < 1e-5
