@@ -239,3 +239,41 @@ def test_too_small_box_is_refused_and_the_library_stays_usable():
                                 SolverMethod.BornSerise, 1.0, 1.0, Options().Hip().Order(3))
     s.Compute()
     assert np.all(np.isfinite(s.GetPsi())) and np.abs(s.GetPsi()).max() > 0
+
+
+def test_c4_full_job_fp32_tracks_complex128():
+    """C4 as configured: 128^3 on [-32,32)^3, 32 energies in one job with
+    complex64 iterates, Vellekoop, slow 0.5, 200 iterations.  Energies 0 and
+    31 against complex128 single-energy jobs (pinned to the oracle by
+    test_c4_grid_fp64_matches_oracle) after the same iterations: dpsi within
+    1e-5 relative L2, the forward amplitude within 1e-6.  Energy 31 alone in
+    an FP32 job equals its slot of the 32-energy launch bit for bit."""
+    from qusim_amd.api import Options, QuPerturbation3D, SolverMethod
+    from qusim_amd.workloads import c4_energies, c4_potential
+    n, L = 128, 32.0
+    V, Es = c4_potential(n=n, L=L), c4_energies()
+    assert len(Es) == 32
+
+    def job(prec, energies):
+        o = Options().Hip().Order(1).Preconditional(True).VellekoopPreconditioner().Slow(0.5)
+        if prec == 32:
+            o.Precision("mixed32")
+        return QuPerturbation3D().init(V, -L, L, n, -L, L, n, -L, L, n, energies, 0.0, 0, 0, 1,
+                                       SolverMethod.BornSerise, 1.0, 1.0, o)
+
+    s32 = job(32, Es)
+    s32.Compute(max_iter=200, tol=0.0)
+    assert list(s32.iters) == [200] * 32
+    for e in (0, 31):
+        s64 = job(64, Es[e])
+        s64.Compute(max_iter=200, tol=0.0)
+        d = rel_l2(s32.GetPsi(e), s64.GetPsi())
+        a64, a32 = s64.ScatteringAmplitude(0, 0, 1), s32.ScatteringAmplitude(0, 0, 1, e)
+        print("C4 energy", e, "dpsi rel L2", d, "forward amplitude", abs(a32 - a64) / abs(a64))
+        assert d < 1e-5 and abs(a32 - a64) < 1e-6 * abs(a64)
+        s64.close()
+    one = job(32, Es[31])
+    one.Compute(max_iter=200, tol=0.0)
+    assert np.array_equal(one.GetPsi(), s32.GetPsi(31))
+    one.close()
+    s32.close()
