@@ -280,8 +280,11 @@ class Evolver:
         for k, v in mine.items():
             if h.get(k) != v:
                 raise ValueError("snapshot %s %r, evolver %s %r" % (k, h.get(k), k, v))
+        step = h.get("step")
+        if not isinstance(step, int) or step < 0:
+            raise ValueError("snapshot step %r is not a step count" % (step,))
         self.psi.upload(psi)
-        self.fStep = int(h["step"])
+        self.fStep = step
 
     def close(self):
         if getattr(self, "_job", None):

@@ -64,10 +64,17 @@ def read(path):
             h = json.loads(hb.decode("utf-8"))
         except ValueError as e:
             raise ValueError("%s: bad header (%s)" % (path, e)) from None
+        if not isinstance(h, dict):
+            raise ValueError("%s: unsupported header (not a JSON object)" % path)
         if h.get("format") != FORMAT or h.get("dtype") != "<c16":
             raise ValueError("%s: unsupported format %r / dtype %r"
                              % (path, h.get("format"), h.get("dtype")))
-        shape = tuple(int(s) for s in h["shape"])
+        shape = h.get("shape")
+        if (not isinstance(shape, list) or not shape
+                or not all(isinstance(s, int) and s >= 1 for s in shape)
+                or not isinstance(h.get("crc32"), int)):
+            raise ValueError("%s: header lacks a valid shape or crc32" % path)
+        shape = tuple(shape)
         n = int(np.prod(shape))
         data = f.read(16 * n + 1)
     if len(data) != 16 * n:

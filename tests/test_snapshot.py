@@ -77,3 +77,15 @@ def test_foreign_files_are_rejected(tmp_path):
     open(p, "wb").write(snapshot.MAGIC + struct.pack("<Q", len(hb)) + hb + b"\0" * 8)
     with pytest.raises(ValueError, match="unsupported"):
         snapshot.read(p)
+
+
+def test_headers_without_shape_or_checksum_are_rejected(tmp_path):
+    p = str(tmp_path / "f.qpsi")
+    for hb in (b'{"format": 1, "dtype": "<c16", "crc32": 0}',
+               b'{"format": 1, "dtype": "<c16", "shape": [0], "crc32": 0}',
+               b'{"format": 1, "dtype": "<c16", "shape": [2, "x"], "crc32": 0}',
+               b'{"format": 1, "dtype": "<c16", "shape": [1]}',
+               b'[1, 2, 3]'):
+        open(p, "wb").write(snapshot.MAGIC + struct.pack("<Q", len(hb)) + hb + b"\0" * 16)
+        with pytest.raises(ValueError):
+            snapshot.read(p)
