@@ -174,8 +174,10 @@ struct Pow2Line {
 
     // twiddle + butterfly for a pass of radix R after NS points combined:
     // one FP64 table load per butterfly, the powers w^r by FP64
-    // multiplication (error ~r ulp, far below the 1e-12 bar); FP32 data
-    // takes each power as a float-float pair.
+    // multiplication (error ~r ulp, far below the 1e-12 bar).  FP32 data is
+    // widened, multiplied in FP64 and rounded once: no rounded twiddle is
+    // applied again at every step, and it takes half the instructions of a
+    // float-float product.
     template <int R, int NS, bool INV>
     static __device__ __forceinline__ void pass(cx<T> *v, int t, const c128 *__restrict__ tw)
     {
@@ -194,17 +196,13 @@ struct Pow2Line {
                         if (r + 1 < R) w = w * w1;
                     }
                 } else {
-                    // the same FP64 powers, each split into a float-float
-                    // pair (one table load per butterfly, no rounded
-                    // constant applied twice)
+                    // the same FP64 powers; the product rounds once
                     const c128 w1 = tw[step];
                     c128 w = w1;
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
-                        c64 wh, wl;
-                        split_ff(w, wh, wl);
-                        v[u * R + r] = INV ? mulconj_ff(v[u * R + r], wh, wl)
-                                           : mul_ff(v[u * R + r], wh, wl);
+                        const c128 x = cvt<double>(v[u * R + r]);
+                        v[u * R + r] = cvt<float>(INV ? mulconj(x, w) : x * w);
                         if (r + 1 < R) w = w * w1;
                     }
                 }

@@ -221,13 +221,8 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF, FftV<T, N>::V>::THREADS))
                 const int k2 = P::template in_pos<RZ>(t, u, r);
                 const size_t m = m1 * (size_t)k2;  // < n1 * n2
                 cx<T> a = v[u * RZ + r];
-                if constexpr (sizeof(T) == 8) {
-                    a = a * twn[m];
-                } else {
-                    c64 h, lo;
-                    split_ff(twn[m], h, lo);
-                    a = mul_ff(a, h, lo);
-                }
+                if constexpr (sizeof(T) == 8) a = a * twn[m];
+                else a = cvt<float>(cvt<double>(a) * twn[m]);   // one rounding
                 if constexpr (INV) a.y = -a.y;
                 out[base + (size_t)k2 * SI] = a;
             }

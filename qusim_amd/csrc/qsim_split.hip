@@ -15,9 +15,9 @@
 // sequence unfused (FFT plans + pointwise kernels).
 //
 // Precision 64: complex128 everywhere.  Precision 32 (FP32-mixed): psi is
-// stored and transformed in complex64; every phase factor is built in FP64
-// and applied as hi + lo FP32 parts, so the rounding of a factor is never
-// repeated step after step (only the product is rounded).
+// stored and transformed in complex64; every phase factor and inter-pass
+// twiddle is FP64 and its product with the data is formed in FP64 and rounded
+// once, so the rounding of a factor is never repeated step after step.
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -41,19 +41,13 @@ struct SplitDev {
     const c128 *ky[2];     // kinetic factor along y (2D)
 };
 
-// v * f with f an FP64 factor: FP64 multiply, or for FP32 data the factor
-// split into hi + lo FP32 parts.  The FP32 form folds the lo products into
-// the FMAs before the last rounding (mul_ff).  Rounding v * hi first and then
-// adding v * lo lost the sub-ulp part of a near-identity factor (tiny V dt)
-// the same way at every step.  At 2048^2, dt = 0.1, that bias alone was half
-// of the 1.1e-5 psi error after 1000 steps; with mul_ff the error is 5.2e-6.
+// v * f with f an FP64 factor.  FP32 data is widened, multiplied in FP64 and
+// rounded once, so a near-identity factor (tiny V dt) keeps its sub-ulp part
+// instead of losing it the same way at every step.  A float-float product
+// that rounded v * hi before adding v * lo lost it: at 2048^2, dt = 0.1, that
+// bias was half of a 1.1e-5 psi error after 1000 steps.
 __device__ __forceinline__ c128 apply(c128 v, c128 f) { return v * f; }
-__device__ __forceinline__ c64 apply(c64 v, c128 f)
-{
-    c64 hi, lo;
-    split_ff(f, hi, lo);
-    return mul_ff(v, hi, lo);
-}
+__device__ __forceinline__ c64 apply(c64 v, c128 f) { return cvt<float>(cvt<double>(v) * f); }
 
 template <typename TO, typename TI>
 __device__ __forceinline__ cx<TO> conv(cx<TI> a)
