@@ -277,12 +277,20 @@ constexpr size_t PLANE_LDS =
     (size_t)(PLANE_N * PLANE_LS > PLANE_N * PLANE_TS ? PLANE_N * PLANE_LS : PLANE_N * PLANE_TS) *
     sizeof(c64);
 
+// The plane kernels sit at the 128-register limit of a 1024-thread
+// workgroup.  With the recurrence form of the twiddle powers the END pass
+// spilled 28 B per lane (Vellekoop, Hao1); the product-tree form leaves it
+// without spills (C4: +2.5%).  Hao2's END spills more with the tree (76 B
+// against 12 B), so it keeps the recurrence.
+template <int PRE>
+using PlaneLineFor = Pow2Line<float, PLANE_N, 16, 8, 1, 1, PRE != PRE_HAO2>;
+
 template <int MODE, int PRE>
 __global__ void __launch_bounds__(PLANE_THREADS)
     k_born_plane(const BornDev *__restrict__ jobs, const c64 *__restrict__ Vf, BornGeom geo,
                  const c128 *__restrict__ tw)
 {
-    using P = PlaneLine;
+    using P = PlaneLineFor<PRE>;
     constexpr int N = PLANE_N;
     constexpr int RA = P::RA_, RZ = P::RLAST;
     extern __shared__ unsigned char smem_raw[];

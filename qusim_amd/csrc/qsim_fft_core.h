@@ -146,7 +146,11 @@ __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 // RC > 1).  Thread t < TPL holds E = RMAX elements; for a pass of radix R it
 // owns butterflies j = t + u*TPL, u < E/R, element r of butterfly u in
 // v[u*R + r].  More passes mean smaller E: more threads (waves) per line.
-template <typename T, int N, int RA, int RB, int RC, int RD = 1>
+// TREE: build each butterfly's twiddle powers as a product tree (w^r =
+// w^(r/2) w^(r - r/2), all R-1 powers live) instead of the recurrence
+// w^r = w^(r-1) w (one live power).  Same multiplies, other register
+// allocation; a kernel opts in where that avoids spills.
+template <typename T, int N, int RA, int RB, int RC, int RD = 1, bool TREE = false>
 struct Pow2Line {
     static_assert(RA * RB * RC * RD == N, "radices must multiply to N");
     static_assert(RD == 1 || RC > 1, "a fourth pass needs a third");
@@ -187,7 +191,21 @@ struct Pow2Line {
                 const int j = t + u * TPL;
                 const int k = j & (NS - 1);
                 const int step = k * (N / (NS * R));
-                if constexpr (sizeof(T) == 8) {
+                if constexpr (TREE) {
+                    c128 w[R];
+                    w[1] = tw[step];
+#pragma unroll
+                    for (int r = 2; r < R; ++r) w[r] = w[r / 2] * w[r - r / 2];
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        if constexpr (sizeof(T) == 8) {
+                            v[u * R + r] = INV ? mulconj(v[u * R + r], w[r]) : v[u * R + r] * w[r];
+                        } else {
+                            const c128 x = cvt<double>(v[u * R + r]);
+                            v[u * R + r] = cvt<float>(INV ? mulconj(x, w[r]) : x * w[r]);
+                        }
+                    }
+                } else if constexpr (sizeof(T) == 8) {
                     const cx<T> w1 = tw[step];
                     cx<T> w = w1;
 #pragma unroll
