@@ -16,7 +16,7 @@ namespace qsim {
 // ------------------------------------------------------------ twiddle tables
 // tw[m] = exp(-2 pi i m / n), m < n, from x87 long double, rounded once to
 // double.  Kernels of both precisions read this FP64 table (FP32 kernels
-// split each power into a float-float pair in registers).
+// widen each element, multiply by the FP64 power and round once).
 static std::mutex g_tw_mu;
 static std::map<std::pair<int, size_t>, void *> g_tw;
 

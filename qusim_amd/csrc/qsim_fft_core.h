@@ -43,18 +43,6 @@ __device__ __forceinline__ c64 mul_ff(c64 a, c64 h, c64 l)
     const float sr = a.x * l.x - a.y * l.y, si = a.x * l.y + a.y * l.x;
     return mk(fmaf(a.x, h.x, fmaf(-a.y, h.y, sr)), fmaf(a.x, h.y, fmaf(a.y, h.x, si)));
 }
-// float-float pair of an FP64 complex: hi = nearest float, lo = the rest
-__device__ __forceinline__ void split_ff(c128 w, c64 &h, c64 &l)
-{
-    h = mk((float)w.x, (float)w.y);
-    l = mk((float)(w.x - (double)h.x), (float)(w.y - (double)h.y));
-}
-// a * conj(h + l)
-__device__ __forceinline__ c64 mulconj_ff(c64 a, c64 h, c64 l)
-{
-    const float sr = a.x * l.x + a.y * l.y, si = a.y * l.x - a.x * l.y;
-    return mk(fmaf(a.x, h.x, fmaf(a.y, h.y, sr)), fmaf(a.y, h.x, fmaf(-a.x, h.y, si)));
-}
 
 // a * exp(sgn * 2 pi i * k / m), sgn = +1 for INV, -1 forward; k, m are
 // compile-time after unrolling, so every branch folds away.
