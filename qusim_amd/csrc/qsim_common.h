@@ -117,6 +117,14 @@ __device__ __forceinline__ cx<T> ld_g(const cx<T> *p)
     const GT *q = (const GT *)&p->x;
     return mk(q[0], q[1]);
 }
+// Non-temporal load of data read once per sweep (psi lines).
+template <typename T>
+__device__ __forceinline__ cx<T> ld_nt(const cx<T> *p)
+{
+    using GT = __attribute__((address_space(1))) const T;
+    const GT *q = (const GT *)&p->x;
+    return mk(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+}
 __device__ __forceinline__ double ld_g(const double *p)
 {
     using GT = __attribute__((address_space(1))) const double;

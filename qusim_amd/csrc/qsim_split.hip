@@ -55,6 +55,19 @@ __device__ __forceinline__ cx<TO> conv(cx<TI> a)
     return cvt<TO>(a);
 }
 
+// Sweep input lines are read once per sweep.  Up to 1024-point lines a
+// non-temporal load keeps them from displacing E_V, the kinetic tables and
+// the twiddles in L2, which the next step reads again (1024^2: psi and E_V
+// are 32 MiB together).  Measured: 1024^2 FP64 4.22 -> 4.45-4.49e10
+// grid-pt-steps/s, FP32 4.73 -> 5.26e10; at 2048^2 FP64 the same loads cost
+// 10%, so longer lines load normally.
+template <int N, typename T>
+__device__ __forceinline__ cx<T> ld_line(const cx<T> *p)
+{
+    if constexpr (N <= 1024) return ld_nt(p);
+    else return ld_g(p);
+}
+
 // ---------------------------------------------------------------- 2D sweeps
 template <typename T, int N, int MODE, int V>
 __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
@@ -93,7 +106,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
                 v[u * RZ + r] =
-                    active ? ld_g(work + base + P::template in_pos<RZ>(t, u, r)) : mk<T>(0, 0);
+                    active ? ld_line<N>(work + base + P::template in_pos<RZ>(t, u, r)) : mk<T>(0, 0);
         // the potential factors do not depend on the transform: issue their
         // loads first so they land while the inverse FFT runs
         constexpr bool PREFETCH = P::E <= 16;
@@ -167,7 +180,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
 #pragma unroll
         for (int r = 0; r < RA; ++r)
             v[u * RA + r] =
-                active ? ld_g(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
+                active ? ld_line<N>(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
                        : mk<T>(0, 0);
     // the kinetic factors depend only on the thread's output positions:
     // issue their loads with the data so they land during the forward FFT
