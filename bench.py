@@ -145,8 +145,8 @@ PROBE_VARIANTS = (6, 4)         # row / column launch variants of the traffic pr
 
 def probe_kernels(rv, cv):
     return (("copy", "qsim_hip_copy"),
-            ("row_sweep", "k_row_sweep<double, 1024, 1, %d>" % rv),
-            ("col_sweep", "k_col_sweep<double, 1024, %d>" % cv))
+            ("row_sweep", "k_row_sweep<double, 1024, 1, %d," % rv),
+            ("col_sweep", "k_col_sweep<double, 1024, %d," % cv))
 
 
 def batch_probe(steps, njobs, variants, n=1024):

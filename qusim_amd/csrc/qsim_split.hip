@@ -55,21 +55,28 @@ __device__ __forceinline__ cx<TO> conv(cx<TI> a)
     return cvt<TO>(a);
 }
 
-// Sweep input lines are read once per sweep.  Up to 1024-point lines a
-// non-temporal load keeps them from displacing E_V, the kinetic tables and
-// the twiddles in L2, which the next step reads again (1024^2: psi and E_V
-// are 32 MiB together).  Measured: 1024^2 FP64 4.22 -> 4.45-4.49e10
-// grid-pt-steps/s, FP32 4.73 -> 5.26e10; at 2048^2 FP64 the same loads cost
-// 10%, so longer lines load normally.
-template <int N, typename T>
+// Sweep input lines are read once per sweep.  A non-temporal load keeps them
+// from displacing E_V, the kinetic tables and the twiddles in L2, which the
+// next step reads again.  Measured per launch: with at least 8 MiB of lines
+// of up to 1024 points it pays (1024^2 FP64 4.22 -> 4.45e10 grid-pt-steps/s,
+// FP32 4.73 -> 5.26e10, 16 batched 256^2 FP32 jobs 5.2 -> 6.4e10); smaller
+// launches lose 3-7% (their lines stay in L2 anyway) and 2048-point lines
+// lose 10% at 2048^2 FP64.  The sweeps take the form as a template argument
+// (a runtime branch around the loads lost the gain) and the host picks it
+// per launch (line_nt).
+template <bool NT, typename T>
 __device__ __forceinline__ cx<T> ld_line(const cx<T> *p)
 {
-    if constexpr (N <= 1024) return ld_nt(p);
+    if constexpr (NT) return ld_nt(p);
     else return ld_g(p);
+}
+static bool line_nt(int n, size_t line_bytes_per_launch)
+{
+    return n <= 1024 && line_bytes_per_launch >= ((size_t)8 << 20);
 }
 
 // ---------------------------------------------------------------- 2D sweeps
-template <typename T, int N, int MODE, int V>
+template <typename T, int N, int MODE, int V, bool NT>
 __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
     k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
                 const c128 *__restrict__ tw)
@@ -106,7 +113,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
                 v[u * RZ + r] =
-                    active ? ld_line<N>(work + base + P::template in_pos<RZ>(t, u, r)) : mk<T>(0, 0);
+                    active ? ld_line<NT>(work + base + P::template in_pos<RZ>(t, u, r)) : mk<T>(0, 0);
         // the potential factors do not depend on the transform: issue their
         // loads first so they land while the inverse FFT runs
         constexpr bool PREFETCH = P::E <= 16;
@@ -154,7 +161,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
 }
 
 // Column sweep over ncols columns of an nx(=N) x ncols array.
-template <typename T, int N, int V>
+template <typename T, int N, int V, bool NT>
 __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
     k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
                 const c128 *__restrict__ tw)
@@ -180,7 +187,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
 #pragma unroll
         for (int r = 0; r < RA; ++r)
             v[u * RA + r] =
-                active ? ld_line<N>(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
+                active ? ld_line<NT>(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
                        : mk<T>(0, 0);
     // the kinetic factors depend only on the thread's output positions:
     // issue their loads with the data so they land during the forward FFT
@@ -432,7 +439,9 @@ int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const voi
                hipStream_t s)
 {
     using LP = P2Launch<T, N, false, V>;
-    auto fn = k_row_sweep<T, N, MODE, V>;
+    auto fn = k_row_sweep<T, N, MODE, V, false>;
+    if constexpr (N <= 1024)
+        if (line_nt(N, (size_t)njobs * nrows * N * sizeof(cx<T>))) fn = k_row_sweep<T, N, MODE, V, true>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     dim3 grid((unsigned)((nrows + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, nrows, iev,
@@ -446,7 +455,9 @@ int launch_col(const SplitDev *jobs, int njobs, size_t ncols, int ik, const void
                hipStream_t s)
 {
     using LP = P2Launch<T, N, true, V>;
-    auto fn = k_col_sweep<T, N, V>;
+    auto fn = k_col_sweep<T, N, V, false>;
+    if constexpr (N <= 1024)
+        if (line_nt(N, (size_t)njobs * ncols * N * sizeof(cx<T>))) fn = k_col_sweep<T, N, V, true>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     dim3 grid((unsigned)((ncols + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, ncols, ik,
