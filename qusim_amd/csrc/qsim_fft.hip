@@ -58,7 +58,12 @@ static int launch_p2_inst(const void *in, void *out, size_t S, size_t nlines, co
                           double scale, hipStream_t s)
 {
     using LP = P2Launch<T, N, LF, FftV<T, N>::V>;
-    auto fn = k_fft_p2<T, N, LF, INV>;
+    auto fn = k_fft_p2<T, N, LF, INV, false>;
+    if constexpr (N >= 128 && N <= 1024) {
+        const size_t bytes = nlines * N * sizeof(cx<T>);   // one pass over the array
+        if (bytes >= ((size_t)8 << 20) && bytes <= ((size_t)16 << 20))
+            fn = k_fft_p2<T, N, LF, INV, true>;
+    }
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t blocks = (nlines + LP::L - 1) / LP::L;
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(LP::THREADS), LP::LDS_BYTES, s,
