@@ -172,6 +172,27 @@ def batch_probe(steps, njobs, variants, n=1024):
     log("batch probe: %d FP32-mixed %d^2 jobs, %d steps, variants %s" % (njobs, n, steps, variants))
 
 
+def fft_probe(reps):
+    """FourierTransform plans alone, for rocprofv3: `reps` forward 2D/3D
+    transforms each of 1024^2 and 2048^2 (complex128 and complex64) and 128^3
+    (both), out of place, from a constant input; no timing."""
+    from qusim_amd import _native as Nn
+    from qusim_amd.fft import FourierTransform
+    Nn.init(0)
+    for dims in ((1024, 1024), (2048, 2048), (128, 128, 128)):
+        for prec in (64, 32):
+            dt = np.complex128 if prec == 64 else np.complex64
+            a = Nn.DeviceArray.from_host(np.ones(dims, dt))
+            b = Nn.DeviceArray(dims, dt)
+            f = FourierTransform(dims, False, prec)
+            for _ in range(reps):
+                f.transform(a, b)
+            Nn.synchronize()
+            a.close()
+            b.close()
+    log("fft probe: %d transforms per shape and precision" % reps)
+
+
 def traffic_probe(steps):
     """Workload of the rocprofv3 --pmc passes: 3 calibration copies of
     PROBE_COPY_N complex128 (known bytes), then `steps` C2 steps with fixed
@@ -459,6 +480,8 @@ def main():
                     help="run only NJOBS batched FP32-mixed 1024^2 jobs (for rocprofv3) and exit")
     ap.add_argument("--probe-variants", type=int, nargs=2, default=(7, 7))
     ap.add_argument("--probe-n", type=int, default=1024, choices=(1024, 2048))
+    ap.add_argument("--fft-probe", action="store_true",
+                    help="run only FourierTransform plans of 1024^2, 2048^2, 128^3 (rocprofv3)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc outputs (CPU)")
     args = ap.parse_args()
@@ -470,6 +493,9 @@ def main():
         return
     if args.batch_probe:
         batch_probe(args.probe_steps, args.batch_probe, tuple(args.probe_variants), args.probe_n)
+        return
+    if args.fft_probe:
+        fft_probe(max(args.probe_steps, 1))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU and has no multi-GPU path: requests and energies "
