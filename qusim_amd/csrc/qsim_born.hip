@@ -159,11 +159,24 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *slots)
     }
 }
 
+// Per-energy arrays (tmp, dpsi) stream through each pass once per
+// iteration, while V and the tables are read again by every energy.  With at
+// least 8 MiB of them per launch, non-temporal loads keep V and the tables in
+// L2 (C4: 3.01 -> 3.14e4 energy-iterations/s with the plane sweep); smaller
+// launches keep normal loads, as the split sweeps do.
+template <bool NT, typename T>
+__device__ __forceinline__ cx<T> ld_stream(const cx<T> *p)
+{
+    if constexpr (NT) return ld_nt(p);
+    else return ld_g(p);
+}
+static bool stream_nt(size_t bytes_per_launch) { return bytes_per_launch >= ((size_t)8 << 20); }
+
 // Inner sweep: lines along the last axis.  Line g -> outer indices
 // (i, j) = (g / nmid, g % nmid) in 3D, i = g in 2D.  The operands of the
 // update (dpsi, V) are loaded before the inverse transform, so their latency
 // hides behind it.
-template <typename T, int N, int MODE, int PRE>
+template <typename T, int N, int MODE, int PRE, bool NT>
 __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
     k_born_inner(const BornDev *__restrict__ jobs, const c128 *__restrict__ V, BornGeom geo,
                  const c128 *__restrict__ tw)
@@ -193,7 +206,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
         for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
-                v[u * RZ + r] = active ? ld_g(tmp + base + P::template in_pos<RZ>(t, u, r))
+                v[u * RZ + r] = active ? ld_stream<NT>(tmp + base + P::template in_pos<RZ>(t, u, r))
                                        : mk<T>(0, 0);
     }
     c128 vv[P::E];
@@ -204,7 +217,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
         for (int r = 0; r < RA; ++r) {
             const int k = P::template in_pos<RA>(t, u, r);
             vv[u * RA + r] = active ? ld_g(V + base + k) : mk(0.0, 0.0);
-            dd[u * RA + r] = active ? ld_g(dp + base + k) : mk<T>(0, 0);
+            dd[u * RA + r] = active ? ld_stream<NT>(dp + base + k) : mk<T>(0, 0);
         }
     if constexpr (MODE != BA_START) P::inverse(v, t, lds + l * LP::LSTR, tw);
     // per-line factors of the separable tables; the inner index varies below
@@ -391,19 +404,6 @@ __global__ void __launch_bounds__(PLANE_THREADS)
         }
     }
 }
-
-// Per-energy arrays (tmp, dpsi) stream through each pass once per
-// iteration, while V and the tables are read again by every energy.  With at
-// least 8 MiB of them per launch, non-temporal loads keep V and the tables in
-// L2 (C4: 3.01 -> 3.14e4 energy-iterations/s with the plane sweep); smaller
-// launches keep normal loads, as the split sweeps do.
-template <bool NT, typename T>
-__device__ __forceinline__ cx<T> ld_stream(const cx<T> *p)
-{
-    if constexpr (NT) return ld_nt(p);
-    else return ld_g(p);
-}
-static bool stream_nt(size_t bytes_per_launch) { return bytes_per_launch >= ((size_t)8 << 20); }
 
 // Outer sweep: lines along axis 0, stride S = n1*n2, G0 between FFT and IFFT.
 template <typename T, int N, bool NT>
@@ -608,9 +608,13 @@ int launch_inner_t(const BornDev *tab, int nact, const c128 *V, const BornGeom &
                    const void *tw, hipStream_t s)
 {
     using LP = P2Launch<T, N, false, BornV<N>::V>;
-    auto fn = k_born_inner<T, N, MODE, PRE>;
-    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     size_t nlines = geo.n0 * geo.nmid;
+    // V is what the hint protects, and only other energies of the same
+    // launch re-read it soon: complex128 128^3 x 8 energies 1.54 -> 1.61e4
+    // energy-iterations/s, one energy unchanged (-1%) so it keeps plain loads
+    const bool nt = nact >= 2 && stream_nt((size_t)nact * nlines * N * sizeof(cx<T>));
+    auto fn = nt ? k_born_inner<T, N, MODE, PRE, true> : k_born_inner<T, N, MODE, PRE, false>;
+    QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
     dim3 grid((unsigned)((nlines + LP::L - 1) / LP::L), (unsigned)nact);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, tab, V, geo,
                        (const c128 *)tw);
