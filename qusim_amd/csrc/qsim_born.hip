@@ -298,7 +298,7 @@ constexpr size_t PLANE_LDS =
 template <int PRE>
 using PlaneLineFor = Pow2Line<float, PLANE_N, 16, 8, 1, 1, PRE != PRE_HAO2>;
 
-template <int MODE, int PRE>
+template <int MODE, int PRE, bool NT>
 __global__ void __launch_bounds__(PLANE_THREADS)
     k_born_plane(const BornDev *__restrict__ jobs, const c64 *__restrict__ Vf, BornGeom geo,
                  const c128 *__restrict__ tw)
@@ -322,12 +322,13 @@ __global__ void __launch_bounds__(PLANE_THREADS)
     // dpsi at the z-line positions the update visits, loaded before the
     // inverse transforms so its latency hides behind them (V follows later:
     // prefetching both would spill at 128 registers).  dpsi and tmp are this
-    // energy's and stream through once (non-temporal loads); the complex64 V
-    // is read by every energy and keeps normal loads, so it stays in L2.
+    // energy's and stream through once (non-temporal loads when the launch
+    // holds several energies, NT); the complex64 V is read by every energy
+    // and keeps normal loads, so it stays in L2.
     c64 dd[P::E];
 #pragma unroll
     for (int r = 0; r < RA; ++r)
-        dd[r] = ld_nt(dp + pbase + (size_t)j * N + P::template in_pos<RA>(t, 0, r));
+        dd[r] = ld_stream<NT>(dp + pbase + (size_t)j * N + P::template in_pos<RA>(t, 0, r));
     c64 v[P::E];
     if constexpr (MODE != BA_START) {
         // y lines, inverse input order
@@ -335,7 +336,7 @@ __global__ void __launch_bounds__(PLANE_THREADS)
         for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
-                v[u * RZ + r] = ld_nt(tmp + pbase + (size_t)P::template in_pos<RZ>(ty, u, r) * N + kz);
+                v[u * RZ + r] = ld_stream<NT>(tmp + pbase + (size_t)P::template in_pos<RZ>(ty, u, r) * N + kz);
         P::inverse(v, ty, lds + kz * PLANE_LS, tw);
         // transpose: y lines (outputs at in_pos<RA>) -> z lines (inputs at in_pos<RZ>)
         __syncthreads();
@@ -655,7 +656,8 @@ template <int MODE, int PRE>
 int launch_plane_t(const BornDev *tab, int nact, const c64 *Vf, const BornGeom &geo,
                    const void *tw, hipStream_t s)
 {
-    auto fn = k_born_plane<MODE, PRE>;
+    // non-temporal per-energy loads when other energies of the launch re-read V
+    auto fn = nact >= 2 ? k_born_plane<MODE, PRE, true> : k_born_plane<MODE, PRE, false>;
     QSIM_TRY(lds_limit((const void *)fn, PLANE_LDS));
     dim3 grid((unsigned)geo.n0, (unsigned)nact);
     hipLaunchKernelGGL(fn, grid, dim3(PLANE_THREADS), PLANE_LDS, s, tab, Vf, geo,
