@@ -40,6 +40,7 @@ def lib():
         L.orc_split2d_time.restype = _D
         L.orc_norm2_range_1d.argtypes = [_VP, _SZ, _D, _D, _D, _D]
         L.orc_norm2_range_1d.restype = _D
+        L.orc_observables.argtypes = [_I, _SZ, _SZ, _D, _D, _D, _D, _D, _D, _I, _VP, _VP, _VP]
         L.orc_born_run.argtypes = [_I, _VP, _VP, _VP, _D, _VP, _D, _D, _I, _I, _D, _I, _VP,
                                    _VP, _VP, _VP, _VP, _I, _I]
         L.orc_plane_wave.argtypes = [_I, _VP, _VP, _VP, _D, _VP, _D, _D, _VP]
@@ -77,6 +78,11 @@ def ref():
                                    _VP, _VP, _VP, _VP, _I]
         R.ref_xsection3d.argtypes = [_VP, _VP, _VP, _D, _D, _D, _VP, _VP, _VP, _D, _D, _D]
         R.ref_xsection3d.restype = _D
+        R.ref_split1d_cv_run.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _I, _I, _VP, _VP, _SZ,
+                                         _SZ, _I]
+        R.ref_split2d_cv_run.argtypes = [_SZ, _SZ, _D, _D, _D, _D, _D, _D, _D, _D, _I, _VP,
+                                         _VP, _SZ, _SZ, _I]
+        R.ref_observables.argtypes = [_I, _SZ, _SZ, _D, _D, _D, _D, _D, _D, _I, _VP, _VP, _VP]
         _ref = R
     return _ref
 
@@ -157,6 +163,20 @@ def norm2_range_1d(psi, x0, x1, a, b):
     return lib().orc_norm2_range_1d(_p(psi), psi.size, x0, x1, a, b)
 
 
+def observables(psi, V, x0, x1, y0=0.0, y1=1.0, mass=1.0, hbar=1.0, boundary=0):
+    """(KinEn, PotEn, Norm2, Xavg) of the evolver state psi (1D or 2D) with
+    the real potential V; Xavg is 0 in 2D."""
+    psi = _c(psi)
+    V = np.ascontiguousarray(np.real(V), dtype=np.float64)
+    out = (ctypes.c_double * 4)()
+    nx = psi.shape[0]
+    ny = psi.shape[1] if psi.ndim == 2 else 1
+    rc = lib().orc_observables(psi.ndim, nx, ny, x0, x1, y0, y1, mass, hbar, boundary,
+                               _p(psi), _p(V), out)
+    assert rc == 0
+    return tuple(out)
+
+
 def born(V, lo, hi, energy, direction, order, ncalls=1, preconditional=True,
          preconditioner=1, slow=1.0, epsilon=0.0, mass=1.0, hbar=1.0, dpsi=None, threads=1):
     """Returns (dpsi, v_eff, epsilon_used, delta_norm)."""
@@ -235,6 +255,41 @@ def ref_split2d(psi, Vreal, x0, x1, y0, y1, dt, nsteps, method=2, mass=1.0, hbar
     ref().ref_split2d_run(nx, ny, x0, x1, y0, y1, dt.real, dt.imag, mass, hbar, method, _p(V),
                           _p(psi), nsteps, batch)
     return psi
+
+
+def ref_split1d_cv(psi, V, x0, x1, dt, nsteps, method=2, boundary=0, mass=1.0, hbar=1.0,
+                   batch=1, fnes=False):
+    """The reference's 1D split step with a complex potential (its complex
+    MulExp device op) and the optional FNES renormalisation of Step()."""
+    psi = _c(psi).copy()
+    V = _c(V)
+    dt = complex(dt)
+    ref().ref_split1d_cv_run(psi.size, x0, x1, dt.real, dt.imag, mass, hbar, method, boundary,
+                             _p(V), _p(psi), nsteps, batch, int(fnes))
+    return psi
+
+
+def ref_split2d_cv(psi, V, x0, x1, y0, y1, dt, nsteps, method=2, mass=1.0, hbar=1.0, batch=1,
+                   fnes=False):
+    psi = _c(psi).copy()
+    V = _c(V)
+    dt = complex(dt)
+    nx, ny = psi.shape
+    ref().ref_split2d_cv_run(nx, ny, x0, x1, y0, y1, dt.real, dt.imag, mass, hbar, method,
+                             _p(V), _p(psi), nsteps, batch, int(fnes))
+    return psi
+
+
+def ref_observables(psi, V, x0, x1, y0=0.0, y1=1.0, mass=1.0, hbar=1.0, boundary=0):
+    psi = _c(psi)
+    V = np.ascontiguousarray(np.real(V), dtype=np.float64)
+    out = (ctypes.c_double * 4)()
+    nx = psi.shape[0]
+    ny = psi.shape[1] if psi.ndim == 2 else 1
+    rc = ref().ref_observables(psi.ndim, nx, ny, x0, x1, y0, y1, mass, hbar, boundary,
+                               _p(psi), _p(V), out)
+    assert rc == 0
+    return tuple(out)
 
 
 def ref_born(V, lo, hi, energy, direction, order, ncalls=1, preconditional=True,

@@ -446,6 +446,78 @@ double orc_norm2_range_1d(const orc_cplx *psi, size_t n, double x0, double x1, d
     return orc_norm2(psi + i1, (size_t)(i2 - i1)) * dx;
 }
 
+/* Evolver observables, out[0..3] = KinEn, PotEn, Norm2 (times dv), Xavg (1D).
+ *   KinEn 1D periodic: SplittingMethod1D::CalKinEn (src/SplittingMethod1D.cpp:94-110):
+ *     sum |FFT psi|^2 fold(i)^2 DT / sum |FFT psi|^2, DT = Dk^2 / 2m (no hbar, as there)
+ *   KinEn 1D wall (:112-128): odd extension to 2N, inverse 2N DFT,
+ *     sum_{i<N} |b_i|^2 i^2 DT / sum_{i<N} |b_i|^2, DT = (hbar pi/(dx N))^2 / 2m
+ *   KinEn 2D: SplittingMethod2D::CalKinEn (src/SplittingMethod2D.cpp:124-141)
+ *   PotEn:    QuEvolverImpl::CalPotEn (src/EvolverImpl.cpp:45-49): sum |psi|^2 V dv / Norm2
+ *   Xavg:     QuEvolver1DImpl::Xavg (src/EvolverImpl1D.cpp:82-87): x0 + dx sum|psi|^2 i / sum|psi|^2
+ */
+int orc_observables(int dim, size_t nx, size_t ny, double x0, double x1, double y0, double y1,
+                    double mass, double hbar, int boundary, const orc_cplx *psi,
+                    const double *V, double *out)
+{
+    if (dim != 1 && dim != 2) return -1;
+    const size_t n = dim == 1 ? nx : nx * ny;
+    const double dx = (x1 - x0) / nx, dy = dim == 1 ? 1.0 : (y1 - y0) / ny;
+    double kin = 0;
+    if (dim == 1 && boundary == 0) {
+        C *ft = (C *)malloc(sizeof(C) * n);
+        orc_fft(1, &n, 0, psi, ft, 1);
+        double Dk = 2 * kPi / (dx * n), DT = Dk * Dk / (2 * mass);
+        double s = 0;
+        for (size_t i = 0; i < n; ++i) {
+            double k = (double)fold_half(i, n);
+            s += cabs2(ft[i]) * (k * k) * DT;
+        }
+        kin = s / orc_norm2(ft, n);
+        free(ft);
+    } else if (dim == 1) {
+        size_t N2 = 2 * n;
+        C *a = (C *)malloc(sizeof(C) * N2), *b = (C *)malloc(sizeof(C) * N2);
+        memcpy(a, psi, sizeof(C) * n);
+        a[0] = cmk(0, 0);
+        a[n] = cmk(0, 0);
+        for (size_t m = 0; m + 1 < n; ++m) a[N2 - 1 - m] = cmk(-a[1 + m].re, -a[1 + m].im);
+        orc_fft(1, &N2, 1, a, b, 1);
+        double Dk = kPi / (dx * n), DT = (Dk * hbar) * (Dk * hbar) / (2 * mass);
+        double s = 0;
+        for (size_t i = 0; i < n; ++i) s += cabs2(b[i]) * ((double)i * (double)i);
+        kin = s * DT / orc_norm2(b, n);
+        free(a);
+        free(b);
+    } else {
+        C *ft = (C *)malloc(sizeof(C) * n);
+        size_t dims[2] = {nx, ny};
+        orc_fft(2, dims, 0, psi, ft, 1);
+        double Dkx = 2 * kPi / (dx * nx), Dky = 2 * kPi / (dy * ny);
+        double DTx = (Dkx * hbar) * (Dkx * hbar) / (2 * mass);
+        double DTy = (Dky * hbar) * (Dky * hbar) / (2 * mass);
+        double s = 0;
+        for (size_t i = 0; i < nx; ++i)
+            for (size_t j = 0; j < ny; ++j) {
+                double ki = (double)fold_half(i, nx), kj = (double)fold_half(j, ny);
+                s += cabs2(ft[i * ny + j]) * (ki * ki * DTx + kj * kj * DTy);
+            }
+        kin = s / orc_norm2(ft, n);
+        free(ft);
+    }
+    const double dv = dx * dy;
+    const double nrm = orc_norm2(psi, n);
+    double pot = 0, idx = 0;
+    for (size_t i = 0; i < n; ++i) {
+        pot += cabs2(psi[i]) * V[i];
+        idx += cabs2(psi[i]) * (double)i;
+    }
+    out[0] = kin;
+    out[1] = pot * dv / (nrm * dv);
+    out[2] = nrm * dv;
+    out[3] = dim == 1 ? x0 + dx * (idx / nrm) : 0;
+    return 0;
+}
+
 /* ---- 2D ---- */
 typedef struct {
     size_t nx, ny, n;

@@ -69,6 +69,15 @@ double orc_split2d_time(size_t nx, size_t ny, double x0, double x1, double y0, d
 double orc_norm2_range_1d(const orc_cplx *psi, size_t n, double x0, double x1,
                           double a, double b);
 
+/* Evolver observables of psi on a 1D (ny ignored) or 2D grid with a real
+ * potential V: out[0] KinEn (src/SplittingMethod1D.cpp:94-128,
+ * src/SplittingMethod2D.cpp:124-141; boundary 1 = infinite wall, 1D only),
+ * out[1] PotEn (src/EvolverImpl.cpp:45-49), out[2] Norm2 times dv
+ * (src/EvolverImpl.cpp:51-54), out[3] Xavg (1D, src/EvolverImpl1D.cpp:82-87). */
+int orc_observables(int dim, size_t nx, size_t ny, double x0, double x1, double y0, double y1,
+                    double mass, double hbar, int boundary, const orc_cplx *psi,
+                    const double *V, double *out);
+
 /* Born series.
  *   dim          : 2 or 3
  *   n[dim], lo[dim], hi[dim] : grid (x_i = lo + i*(hi-lo)/n)

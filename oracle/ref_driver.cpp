@@ -81,10 +81,14 @@ struct Kiss3D : FourierTransform3D {
 };
 
 // Split-operator 1D, driven through QuUpdatePsi (src/SplittingUtils.h:14-64).
+// fVc (complex potential) replaces fV when set: ExpV then runs the
+// reference's complex MulExp (src/Device.h:129, src/DeviceTemplate.h:1478-1486)
+// instead of the real one the evolver uses (src/SplittingMethod1D.cpp:84-89).
 struct Split1D {
     Dev dev;
     Cx* fPsi = nullptr;
     double* fV = nullptr;
+    Cx* fVc = nullptr;
     size_t fBatchSize = 1;
     SolverMethod fSolverMethod;
     uint64_t fStep = 0;
@@ -107,7 +111,8 @@ struct Split1D {
     void ExpV(Cx* psi, double t)
     {
         Complex f = -I / fHbar * fDt * t;
-        dev.MulExp(psi, f, fV, fN);
+        if (fVc) dev.MulExp(psi, f, (Cx const*)fVc, fN);
+        else dev.MulExp(psi, f, fV, fN);
     }
 
     // src/SplittingMethod1D.cpp:40-81
@@ -142,7 +147,9 @@ struct Split1D {
 };
 
 // Split-operator 2D with the cached phase tables of SplittingMethod2D
-// (src/SplittingMethod2D.cpp:38-121).
+// (src/SplittingMethod2D.cpp:38-121).  With a complex potential Vc the
+// tables come from the reference's complex Exp (src/Device.h:104,
+// src/DeviceTemplate.h:1311-1318) instead of the real one.
 struct Split2D {
     Dev dev;
     Cx* fPsi = nullptr;
@@ -153,20 +160,27 @@ struct Split2D {
     std::vector<Cx> ev05, evC1, evC2, et1, etD1, etD2, ft;
     Kiss2D fft, inv_fft;
 
+    template <class VT>
+    void exp_v(Cx* to, VT const* V, Cx f)
+    {
+        dev.Exp(to, V, f, fN);
+    }
+
+    template <class VT>
     Split2D(size_t nx, size_t ny, double dx, double dy, Cx dt, double mass, double hbar,
-            int method, double* V)
+            int method, VT const* V)
         : fNx(nx), fNy(ny), fN(nx * ny), ft(nx * ny), fft(nx, ny, false), inv_fft(nx, ny, true)
     {
         fSolverMethod = method == 4 ? SolverMethod::SplittingMethodO4 : SolverMethod::SplittingMethodO2;
         if (method == 2) {
             ev05.resize(fN);
             ComplexType f = -I / hbar * dt * 0.5;
-            dev.Exp(ev05.data(), V, f, fN);
+            exp_v(ev05.data(), V, f);
         } else {
             evC1.resize(fN);
             evC2.resize(fN);
-            dev.Exp(evC1.data(), V, -I / hbar * dt * SplitingConstants<Real>::C1, fN);
-            dev.Exp(evC2.data(), V, -I / hbar * dt * SplitingConstants<Real>::C2, fN);
+            exp_v(evC1.data(), V, -I / hbar * dt * SplitingConstants<Real>::C1);
+            exp_v(evC2.data(), V, -I / hbar * dt * SplitingConstants<Real>::C2);
         }
         RealType const Dkx = 2 * Pi / (dx * nx);
         RealType const Dky = 2 * Pi / (dy * ny);
@@ -255,6 +269,110 @@ int ref_split2d_run(size_t nx, size_t ny, double x0, double x1, double y0, doubl
     s.fPsi = psi;
     s.fBatchSize = batch;
     for (size_t i = 0; i < nsteps; ++i) QuUpdatePsi(&s);
+    return 0;
+}
+
+// QuEvolverImpl::Step's renormalisation (src/EvolverImpl.cpp:60-63):
+// psi *= 1 / sqrt(Norm2() ), Norm2() = Device::Norm2 * dv (:51-54).
+static void fnes_scale(Dev& dev, Cx* psi, size_t n, double dv)
+{
+    double scale = 1.0 / sqrt(dev.Norm2(psi, n) * dv);
+    dev.Scale(psi, scale, n);
+}
+
+// The same evolutions with a complex potential (the reference's complex
+// Exp / MulExp device ops) and optionally FNES: nsteps Step() calls, each
+// `batch` QuUpdatePsi steps (src/EvolverImpl.cpp:56-68).
+int ref_split1d_cv_run(size_t n, double x0, double x1, double dt_re, double dt_im, double mass,
+                       double hbar, int method, int boundary, const Cx* V, Cx* psi,
+                       size_t nsteps, size_t batch, int fnes)
+{
+    double dx = (x1 - x0) / n;
+    Split1D s(n, dx, Cx(dt_re, dt_im), mass, hbar, method, boundary);
+    std::vector<Cx> v(V, V + n);
+    s.fVc = v.data();
+    s.fPsi = psi;
+    s.fBatchSize = batch;
+    for (size_t i = 0; i < nsteps; ++i) {
+        QuUpdatePsi(&s);
+        if (fnes) fnes_scale(s.dev, psi, n, dx);
+    }
+    return 0;
+}
+
+int ref_split2d_cv_run(size_t nx, size_t ny, double x0, double x1, double y0, double y1,
+                       double dt_re, double dt_im, double mass, double hbar, int method,
+                       const Cx* V, Cx* psi, size_t nsteps, size_t batch, int fnes)
+{
+    std::vector<Cx> v(V, V + nx * ny);
+    const double dx = (x1 - x0) / nx, dy = (y1 - y0) / ny;
+    Split2D s(nx, ny, dx, dy, Cx(dt_re, dt_im), mass, hbar, method, (Cx const*)v.data());
+    s.fPsi = psi;
+    s.fBatchSize = batch;
+    for (size_t i = 0; i < nsteps; ++i) {
+        QuUpdatePsi(&s);
+        if (fnes) fnes_scale(s.dev, psi, nx * ny, dx * dy);
+    }
+    return 0;
+}
+
+// Evolver observables through the reference's device ops.
+// out[0] KinEn, out[1] PotEn, out[2] Norm2 (with dv), out[3] Xavg (1D only).
+//   KinEn 1D: SplittingMethod1D::CalKinEn (src/SplittingMethod1D.cpp:94-128),
+//             periodic (boundary 0) and infinite wall (1)
+//   KinEn 2D: SplittingMethod2D::CalKinEn (src/SplittingMethod2D.cpp:124-141)
+//   PotEn:    QuEvolverImpl::CalPotEn (src/EvolverImpl.cpp:45-49), real V
+//   Norm2:    QuEvolverImpl::Norm2 (src/EvolverImpl.cpp:51-54)
+//   Xavg:     QuEvolver1DImpl::Xavg (src/EvolverImpl1D.cpp:82-87)
+int ref_observables(int dim, size_t nx, size_t ny, double x0, double x1, double y0, double y1,
+                    double mass, double hbar, int boundary, const Cx* psi_in,
+                    const double* V, double* out)
+{
+    try {
+        Dev dev;
+        const size_t n = dim == 1 ? nx : nx * ny;
+        std::vector<Cx> psi(psi_in, psi_in + n);
+        std::vector<double> v(V, V + n);
+        const double dx = (x1 - x0) / nx, dy = dim == 1 ? 1.0 : (y1 - y0) / ny;
+        const double dv = dx * dy;
+        double kin = 0;
+        if (dim == 1 && boundary == 0) {
+            std::vector<Cx> ft(n);
+            kissfft<double> f(n, false);
+            f.transform(psi.data(), ft.data());
+            Real Dk = 2 * Pi / (dx * n);
+            Real DT = Dk * Dk / (2 * mass);
+            kin = dev.Abs2K1D(ft.data(), DT, n) / dev.Norm2(ft.data(), n);
+        } else if (dim == 1) {
+            std::vector<Cx> a(2 * n), b(2 * n);
+            kissfft<double> inv2n(2 * n, true);
+            dev.Copy(a.data(), psi.data(), n);
+            dev.Set(a.data(), 0, 0.);
+            dev.Set(a.data(), n, 0.);
+            dev.CopyReverseMinu(a.data() + 2 * n - 1, a.data() + 1, n - 1);
+            inv2n.transform(a.data(), b.data());
+            Real const Dk = Pi / (dx * n);
+            Real const DT = QuSqr(Dk * hbar) / (2 * mass);
+            kin = dev.Abs2Idx2(b.data(), n) * DT / dev.Norm2(b.data(), n);
+        } else {
+            std::vector<Cx> ft(n);
+            Kiss2D f(nx, ny, false);
+            f.Transform(psi.data(), ft.data());
+            Real Dkx = 2 * Pi / (dx * nx);
+            Real Dky = 2 * Pi / (dy * ny);
+            Real DTx = QuSqr(Dkx * hbar) / (2 * mass);
+            Real DTy = QuSqr(Dky * hbar) / (2 * mass);
+            kin = dev.Abs2K2D(ft.data(), DTx, nx, DTy, ny) / dev.Norm2(ft.data(), n);
+        }
+        const double norm2 = dev.Norm2(psi.data(), n) * dv;
+        Complex pot = dev.Abs2Mul(psi.data(), v.data(), n) * dv;
+        out[0] = kin;
+        out[1] = pot.real() / norm2;
+        out[2] = norm2;
+        out[3] = dim == 1 ? x0 + dx * (dev.Abs2Idx(psi.data(), n) / dev.Norm2(psi.data(), n)) : 0;
+    } catch (...) {
+        return -2;
+    }
     return 0;
 }
 

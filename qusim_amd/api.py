@@ -349,14 +349,32 @@ class Evolver1D(Evolver):
         return self.fX0 + self.fDx * idx
 
     def KinEn(self):
-        """SplittingMethod1D::CalKinEn, periodic (src/SplittingMethod1D.cpp:94-112)."""
-        f = FourierTransform((self.fNx,), False)
-        tmp = N.DeviceArray((self.fNx,), np.complex128)
-        f.transform(self.psi, tmp)
-        Dk = 2 * math.pi / (self.fDx * self.fNx)
-        DT = Dk * Dk / (2 * self._mass)
-        kin = N.reduce1("qsim_hip_abs2_k1d", tmp.ptr, DT, self.fNx)
-        return kin / N.reduce1("qsim_hip_norm2", tmp.ptr, self.fNx)
+        """SplittingMethod1D::CalKinEn (src/SplittingMethod1D.cpp:94-128).
+        Periodic: sum |FFT psi|^2 fold(i)^2 DT / sum |FFT psi|^2 with
+        DT = Dk^2 / 2m (the reference leaves hbar out here).  Infinite wall:
+        the odd extension to 2N through the inverse 2N transform, then
+        sum_{i<N} |b_i|^2 i^2 DT / sum_{i<N} |b_i|^2, DT = (hbar pi/(dx N))^2 / 2m."""
+        n = self.fNx
+        if self._boundary == BoundaryCondition.Period:
+            f = FourierTransform((n,), False)
+            tmp = N.DeviceArray((n,), np.complex128)
+            f.transform(self.psi, tmp)
+            Dk = 2 * math.pi / (self.fDx * n)
+            DT = Dk * Dk / (2 * self._mass)
+            kin = N.reduce1("qsim_hip_abs2_k1d", tmp.ptr, DT, n)
+            return kin / N.reduce1("qsim_hip_norm2", tmp.ptr, n)
+        a = N.DeviceArray((2 * n,), np.complex128)
+        b = N.DeviceArray((2 * n,), np.complex128)
+        zero = N.c128(0.0, 0.0)
+        N.call("qsim_hip_copy", a.ptr, self.psi.ptr, n, None)
+        N.call("qsim_hip_set_elem", a.ptr, 0, zero, None)
+        N.call("qsim_hip_set_elem", a.ptr, n, zero, None)
+        N.call("qsim_hip_copy_reverse_minus", a.ptr + 16 * (2 * n - 1), a.ptr + 16, n - 1, None)
+        FourierTransform((2 * n,), True).transform(a, b)
+        Dk = math.pi / (self.fDx * n)
+        DT = (Dk * self._hbar) ** 2 / (2 * self._mass)
+        kin = N.reduce1("qsim_hip_abs2_idx2", b.ptr, n) * DT
+        return kin / N.reduce1("qsim_hip_norm2", b.ptr, n)
 
 
 class Evolver2D(Evolver):
@@ -464,6 +482,7 @@ class _Perturbation:
                len(self._energies))
         self._job = p.value
         self._buf = N.DeviceArray(self._shape, np.complex128)
+        self._last = None
         self._delta_norm = 0.0
         self.iters = [0] * len(self._energies)
         self.residual = [0.0] * len(self._energies)
@@ -474,15 +493,22 @@ class _Perturbation:
         k = len(self._energies)
         it = (ctypes.c_int * k)()
         rs = (ctypes.c_double * k)()
-        last = self.GetPsi() if self._dim == 2 else None
+        if self._dim == 2:  # the Copy of fLastPsi, src/Perturbation2DImpl.cpp:56
+            if self._last is None:
+                self._last = N.DeviceArray(self._shape, np.complex128)
+            N.call("qsim_hip_born_get_dpsi", self._job, 0, self._last.ptr, None)
         N.call("qsim_hip_born_iterate", self._job, self._order if max_iter is None else max_iter,
                float(tol), it, rs, None)
         self.iters = list(it)
         self.residual = list(rs)
-        if last is not None:  # src/Perturbation2DImpl.cpp:80-82
-            now = self.GetPsi()
-            den = float(np.sum(np.abs(now) ** 2))
-            self._delta_norm = math.sqrt(float(np.sum(np.abs(now - last) ** 2)) / den) if den else 0.0
+        if self._dim == 2:
+            # sqrt(MinusNorm2(psi, last) / Norm2(psi)), src/Perturbation2DImpl.cpp:80-82,
+            # reduced on the device
+            N.call("qsim_hip_born_get_dpsi", self._job, 0, self._buf.ptr, None)
+            n = self._buf.size
+            den = N.reduce1("qsim_hip_norm2", self._buf.ptr, n)
+            num = N.reduce1("qsim_hip_minus_norm2", self._buf.ptr, self._last.ptr, n)
+            self._delta_norm = math.sqrt(num / den) if den else 0.0
         return self
 
     def GetPsi(self, e=0):

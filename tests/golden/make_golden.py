@@ -62,6 +62,37 @@ def main():
     out["s2_V"] = V2
     out["s2_o2_50"] = O.ref_split2d(p2, V2, -30.0, 30.0, -20.0, 25.0, 0.1, 50, method=2)
     out["s2_o4_20"] = O.ref_split2d(p2, V2, -30.0, 30.0, -20.0, 25.0, 0.1, 20, method=4)
+    # Complex potentials (the imaginary absorbing layer of the 2D workloads)
+    # through the reference's complex Exp / MulExp device ops, and the FNES
+    # renormalisation of Step().  1D: the C1 grid with -i 0.1 (d/20)^2 within
+    # d < 20 of either edge.
+    d1 = np.minimum(x + 100.0, 100.0 - x)
+    V1c = V - 1j * np.where(d1 < 20.0, 0.1 * (d1 / 20.0) ** 2, 0.0)
+    out["c1cv_V"] = V1c
+    out["c1cv_o2_1000"] = O.ref_split1d_cv(psi, V1c, -100.0, 100.0, 0.5, 1000, method=2)
+    out["c1cv_o2_150"] = O.ref_split1d_cv(psi, V1c, -100.0, 100.0, 0.5, 150, method=2)
+    out["c1cv_o4_150"] = O.ref_split1d_cv(psi, V1c, -100.0, 100.0, 0.5, 150, method=4)
+    out["c1cv_fnes_o2_120x2"] = O.ref_split1d_cv(psi, V1c, -100.0, 100.0, 0.5, 120, method=2,
+                                                 batch=2, fnes=True)
+    # 2D: 64 x 48 with -i 0.2 (d/8)^2 within d < 8 of the nearest edge
+    dd = np.minimum.reduce([X + 30.0, 30.0 - X, Y + 20.0, 25.0 - Y])
+    V2c = V2 - 1j * np.where(dd < 8.0, 0.2 * (dd / 8.0) ** 2, 0.0)
+    out["s2cv_V"] = V2c
+    out["s2cv_o2_80"] = O.ref_split2d_cv(p2, V2c, -30.0, 30.0, -20.0, 25.0, 0.1, 80, method=2)
+    out["s2cv_o4_30"] = O.ref_split2d_cv(p2, V2c, -30.0, 30.0, -20.0, 25.0, 0.1, 30, method=4)
+    out["s2cv_fnes_o2_40x2"] = O.ref_split2d_cv(p2, V2c, -30.0, 30.0, -20.0, 25.0, 0.1, 40,
+                                                method=2, batch=2, fnes=True)
+    # Evolver observables (KinEn, PotEn, Norm2, Xavg) through the reference's
+    # Abs2K1D / Abs2K2D / Abs2Idx2 / Abs2Mul / Abs2Idx / Norm2 device ops, on
+    # states that are not Gaussian any more
+    s1 = out["c1cv_o2_150"]
+    out["obs_c1_periodic"] = np.array(O.ref_observables(s1, V, -100.0, 100.0, boundary=0))
+    out["obs_c1_wall"] = np.array(O.ref_observables(out["c1_wall_o2_200"], V, -100.0, 100.0,
+                                                    boundary=1))
+    out["obs_c1_hbar_mass"] = np.array(O.ref_observables(s1, V, -100.0, 100.0, mass=1.7,
+                                                         hbar=0.8, boundary=1))
+    out["obs_s2"] = np.array(O.ref_observables(out["s2cv_o2_80"], V2, -30.0, 30.0, -20.0, 25.0,
+                                               mass=1.3, hbar=0.9))
     # Born 2D (48 x 40) and 3D (12 x 10 x 14), each preconditioner
     lo2, hi2 = (-60.0, -50.0), (60.0, 50.0)
     gx = lo2[0] + 120.0 / 48 * np.arange(48)
