@@ -36,8 +36,11 @@ def lib():
         L.orc_split2d_run.argtypes = [_SZ, _SZ, _D, _D, _D, _D, _D, _D, _D, _D, _I, _VP, _I,
                                       _VP, _SZ, _SZ, _I, _I]
         L.orc_split2d_time.argtypes = [_SZ, _SZ, _D, _D, _D, _D, _D, _D, _D, _VP, _I, _VP,
-                                       _SZ, _I]
+                                       _SZ, _I, _I]
         L.orc_split2d_time.restype = _D
+        L.orc_born_time.argtypes = [_I, _VP, _VP, _VP, _D, _VP, _D, _D, _I, _I, _D, _I, _VP,
+                                    _I, _I]
+        L.orc_born_time.restype = _D
         L.orc_norm2_range_1d.argtypes = [_VP, _SZ, _D, _D, _D, _D]
         L.orc_norm2_range_1d.restype = _D
         L.orc_observables.argtypes = [_I, _SZ, _SZ, _D, _D, _D, _D, _D, _D, _I, _VP, _VP, _VP]
@@ -144,13 +147,28 @@ def split2d(psi, V, x0, x1, y0, y1, dt, nsteps, method=2, mass=1.0, hbar=1.0,
     return psi
 
 
-def split2d_time(psi, V, x0, x1, y0, y1, dt, nsteps, complex_v=True, threads=1):
-    """Seconds spent in `nsteps` O2 steps of the oracle loop (set-up excluded)."""
+def split2d_time(psi, V, x0, x1, y0, y1, dt, nsteps, complex_v=True, threads=1,
+                 fft_threads=None):
+    """Seconds spent in `nsteps` O2 steps of the oracle loop (set-up excluded);
+    `threads` for the pointwise ops, `fft_threads` (default: threads) for the
+    FFT lines."""
     psi = _c(psi).copy()
     V = _c(V)
     nx, ny = psi.shape
     return lib().orc_split2d_time(nx, ny, x0, x1, y0, y1, dt, 1.0, 1.0, _p(V), int(complex_v),
-                                  _p(psi), nsteps, threads)
+                                  _p(psi), nsteps, threads,
+                                  threads if fft_threads is None else fft_threads)
+
+
+def born_time(V, lo, hi, energy, direction, iters, preconditional=True, preconditioner=1,
+              slow=1.0, mass=1.0, hbar=1.0, threads=1, fft_threads=None):
+    """Seconds of `iters` oracle Born iterations (the Compute() loop only)."""
+    V = _c(V)
+    s = lib().orc_born_time(V.ndim, _szs(V.shape), _ds(lo), _ds(hi), energy, _ds(direction),
+                            mass, hbar, int(preconditional), preconditioner, slow, iters, _p(V),
+                            threads, threads if fft_threads is None else fft_threads)
+    assert s >= 0
+    return s
 
 
 def norm2(a):

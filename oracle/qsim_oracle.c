@@ -522,7 +522,8 @@ int orc_observables(int dim, size_t nx, size_t ny, double x0, double x1, double 
 typedef struct {
     size_t nx, ny, n;
     C *ev_half, *ev_c1, *ev_c2, *et_1, *et_d1, *et_d2, *ft;
-    int threads;
+    int threads;      /* pointwise ops */
+    int fft_threads;  /* lines of the 2D FFT */
 } split2d;
 
 /* SplittingMethod2D::InitExpT, src/SplittingMethod2D.cpp:56-79 */
@@ -539,7 +540,7 @@ static void s2_make_et(C *t, size_t nx, size_t ny, double DTx, double DTy, C alp
 static void s2_fft(split2d *s, const C *in, C *out, int inverse)
 {
     size_t d[2] = {s->nx, s->ny};
-    orc_fft(2, d, inverse, in, out, s->threads);
+    orc_fft(2, d, inverse, in, out, s->fft_threads);
 }
 
 /* SplittingMethod2D::ExpT, src/SplittingMethod2D.cpp:96-121 */
@@ -557,7 +558,7 @@ static void s2_init(split2d *s, size_t nx, size_t ny, double x0, double x1, doub
 {
     split_constants();
     memset(s, 0, sizeof *s);
-    s->nx = nx; s->ny = ny; s->n = nx * ny; s->threads = threads;
+    s->nx = nx; s->ny = ny; s->n = nx * ny; s->threads = threads; s->fft_threads = threads;
     double dx = (x1 - x0) / nx, dy = (y1 - y0) / ny;
     size_t n = s->n;
     s->ft = (C *)malloc(sizeof(C) * n);
@@ -644,10 +645,11 @@ static double now_s(void)
 
 double orc_split2d_time(size_t nx, size_t ny, double x0, double x1, double y0, double y1,
                         double dt, double mass, double hbar, const orc_cplx *V, int complex_v,
-                        orc_cplx *psi, size_t nsteps, int threads)
+                        orc_cplx *psi, size_t nsteps, int threads, int fft_threads)
 {
     split2d s;
     s2_init(&s, nx, ny, x0, x1, y0, y1, cmk(dt, 0), mass, hbar, 2, V, complex_v, threads);
+    s.fft_threads = fft_threads;
     double t0 = now_s();
     for (size_t st = 0; st < nsteps; ++st) s2_step(&s, psi, 2);
     double t1 = now_s();
@@ -730,12 +732,50 @@ static void g0_mul(int dim, const size_t *n, C *psik, double E, double eps, cons
     }
 }
 
+static int born_impl(int dim, const size_t *n, const double *lo, const double *hi,
+                     double energy, const double *direction, double mass, double hbar,
+                     int preconditional, int preconditioner, double slow, int order,
+                     const orc_cplx *Vin, orc_cplx *dpsi, orc_cplx *v_eff, double *eps_inout,
+                     double *delta_norm, int ncalls, int threads, int fft_threads,
+                     double *loop_seconds);
+
 int orc_born_run(int dim, const size_t *n, const double *lo, const double *hi, double energy,
                  const double *direction, double mass, double hbar, int preconditional,
                  int preconditioner, double slow, int order, const orc_cplx *Vin,
                  orc_cplx *dpsi, orc_cplx *v_eff, double *eps_inout, double *delta_norm,
                  int ncalls, int threads)
 {
+    /* the pointwise loops stay serial here (bitwise the results of the
+     * reference's sequential ops); the FFT lines use `threads` */
+    return born_impl(dim, n, lo, hi, energy, direction, mass, hbar, preconditional,
+                     preconditioner, slow, order, Vin, dpsi, v_eff, eps_inout, delta_norm,
+                     ncalls, 1, threads, NULL);
+}
+
+double orc_born_time(int dim, const size_t *n, const double *lo, const double *hi,
+                     double energy, const double *direction, double mass, double hbar,
+                     int preconditional, int preconditioner, double slow, int iters,
+                     const orc_cplx *Vin, int threads, int fft_threads)
+{
+    size_t tot = n[0] * n[1] * (dim == 3 ? n[2] : 1);
+    C *dpsi = (C *)calloc(tot, sizeof(C));
+    double eps = 0, secs = -1;
+    if (born_impl(dim, n, lo, hi, energy, direction, mass, hbar, preconditional,
+                  preconditioner, slow, iters, Vin, dpsi, NULL, &eps, NULL, 1, threads,
+                  fft_threads, &secs) != 0)
+        secs = -1;
+    free(dpsi);
+    return secs;
+}
+
+static int born_impl(int dim, const size_t *n, const double *lo, const double *hi,
+                     double energy, const double *direction, double mass, double hbar,
+                     int preconditional, int preconditioner, double slow, int order,
+                     const orc_cplx *Vin, orc_cplx *dpsi, orc_cplx *v_eff, double *eps_inout,
+                     double *delta_norm, int ncalls, int threads, int fft_threads,
+                     double *loop_seconds)
+{
+    const int nt = threads > 0 ? threads : 1;
     if (dim != 2 && dim != 3) return -1;
     size_t tot = n[0] * n[1] * (dim == 3 ? n[2] : 1);
     double dx[3] = {0, 0, 0};
@@ -789,22 +829,27 @@ int orc_born_run(int dim, const size_t *n, const double *lo, const double *hi, d
     size_t dims[3] = {n[0], n[1], dim == 3 ? n[2] : 1};
     double rn = 1. / (double)tot;
     double dnorm = 0;
+    const double t_start = now_s();
     for (int call = 0; call < ncalls; ++call) {
         memcpy(last, dpsi, sizeof(C) * tot);
         for (int it = 0; it < order; ++it) {
             if (preconditional) {
                 /* PreconditionalBornSeries::Update, src/Perburbation.h:350-423 */
-                for (size_t i = 0; i < tot; ++i) /* op_Add3 */
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+                for (long long i = 0; i < (long long)tot; ++i) /* op_Add3 */
                     tmp[i] = cadd(cmul(cmk(V[i].re, V[i].im + eps), dpsi[i]),
                                   cscale(psi0[i], V[i].re));
-                orc_fft(dim, dims, 0, tmp, psik, threads);
-                g0_mul(dim, n, psik, energy, eps, a, threads);
-                orc_fft(dim, dims, 1, psik, tmp, threads);
+                orc_fft(dim, dims, 0, tmp, psik, fft_threads);
+                g0_mul(dim, n, psik, energy, eps, a, nt);
+                orc_fft(dim, dims, 1, psik, tmp, fft_threads);
                 if (preconditioner == 0 && slow == 1.) {
-                    for (size_t i = 0; i < tot; ++i) dpsi[i] = cscale(tmp[i], rn);
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+                    for (long long i = 0; i < (long long)tot; ++i) dpsi[i] = cscale(tmp[i], rn);
                 } else {
-                    for (size_t i = 0; i < tot; ++i) tmp[i] = cscale(tmp[i], rn);
-                    for (size_t i = 0; i < tot; ++i) {
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+                    for (long long i = 0; i < (long long)tot; ++i) tmp[i] = cscale(tmp[i], rn);
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+                    for (long long i = 0; i < (long long)tot; ++i) {
                         C g;
                         if (preconditioner == 0) {
                             dpsi[i] = cadd(cscale(dpsi[i], 1 - slow), cscale(tmp[i], slow));
@@ -825,13 +870,17 @@ int orc_born_run(int dim, const size_t *n, const double *lo, const double *hi, d
                 }
             } else {
                 /* BornSeries::Update2D/3D, src/Perburbation.h:257-309 */
-                for (size_t i = 0; i < tot; ++i) dpsi[i] = cmul(cadd(dpsi[i], psi0[i]), V[i]);
-                orc_fft(dim, dims, 0, dpsi, psik, threads);
-                g0_mul(dim, n, psik, energy, eps, a, threads);
-                for (size_t i = 0; i < tot; ++i) psik[i] = cscale(psik[i], rn);
-                orc_fft(dim, dims, 1, psik, dpsi, threads);
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+                for (long long i = 0; i < (long long)tot; ++i)
+                    dpsi[i] = cmul(cadd(dpsi[i], psi0[i]), V[i]);
+                orc_fft(dim, dims, 0, dpsi, psik, fft_threads);
+                g0_mul(dim, n, psik, energy, eps, a, nt);
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+                for (long long i = 0; i < (long long)tot; ++i) psik[i] = cscale(psik[i], rn);
+                orc_fft(dim, dims, 1, psik, dpsi, fft_threads);
             }
         }
+        if (loop_seconds) *loop_seconds = now_s() - t_start;
         double num = 0, den = 0;
         for (size_t i = 0; i < tot; ++i) num += cabs2(csub(dpsi[i], last[i]));
         den = orc_norm2(dpsi, tot);

@@ -59,11 +59,14 @@ int orc_split2d_run(size_t nx, size_t ny, double x0, double x1, double y0, doubl
                     const orc_cplx *V, int complex_v, orc_cplx *psi,
                     size_t nsteps, size_t batch, int fnes, int threads);
 
-/* Time the 2D step loop only (no set-up), returns seconds for nsteps steps. */
+/* Time the 2D O2 step loop only (no set-up), returns seconds for nsteps
+ * steps.  threads: the pointwise ops (Mul, Scale); fft_threads: the lines of
+ * each 2D transform.  The reference's CpuPar threads only the former (its
+ * KISS FFT is serial): threads = cores, fft_threads = 1. */
 double orc_split2d_time(size_t nx, size_t ny, double x0, double x1, double y0, double y1,
                         double dt, double mass, double hbar,
                         const orc_cplx *V, int complex_v, orc_cplx *psi,
-                        size_t nsteps, int threads);
+                        size_t nsteps, int threads, int fft_threads);
 
 /* Observables of the 1D evolver, src/EvolverImpl1D.cpp:82-101. */
 double orc_norm2_range_1d(const orc_cplx *psi, size_t n, double x0, double x1,
@@ -96,6 +99,14 @@ int orc_born_run(int dim, const size_t *n, const double *lo, const double *hi,
                  int preconditional, int preconditioner, double slow, int order,
                  const orc_cplx *V, orc_cplx *dpsi, orc_cplx *v_eff,
                  double *eps_inout, double *delta_norm, int ncalls, int threads);
+
+/* Seconds of `iters` Born iterations (the Compute() loop only, set-up and
+ * the absorbing layer excluded) from dpsi = 0; threads / fft_threads as for
+ * orc_split2d_time.  Returns a negative value on bad arguments. */
+double orc_born_time(int dim, const size_t *n, const double *lo, const double *hi,
+                     double energy, const double *direction, double mass, double hbar,
+                     int preconditional, int preconditioner, double slow, int iters,
+                     const orc_cplx *V, int threads, int fft_threads);
 
 /* Plane wave psi0 = exp(i k.r), src/ScatteringSolverImpl.cpp:272-314. */
 int orc_plane_wave(int dim, const size_t *n, const double *lo, const double *hi,

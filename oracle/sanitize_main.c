@@ -66,6 +66,12 @@ static void splits(void)
     RUN(orc_norm2_range_1d(psi, n, -10, 10, -20, 20));   /* range beyond the grid */
     RUN(orc_norm2_range_1d(psi, n, -10, 10, 5, -5));     /* empty range */
     RUN(orc_norm2_range_1d(psi, n, -10, 10, 0, 10));
+    {
+        double vr[n], out[4];
+        for (size_t i = 0; i < n; ++i) vr[i] = V[i].re;
+        for (int boundary = 0; boundary < 2; ++boundary)
+            RUN(orc_observables(1, n, 1, -10, 10, 0, 1, 1.3, 0.7, boundary, psi, vr, out));
+    }
     free(V);
     free(psi);
     const size_t nx = 16, ny = 32;
@@ -75,7 +81,12 @@ static void splits(void)
         for (int cv = 0; cv < 2; ++cv)
             RUN(orc_split2d_run(nx, ny, -8, 8, -4, 4, 0.05, 0.0, 1, 1, method, V, cv, psi, 2, 3,
                                 cv, 2));
-    RUN(orc_split2d_time(nx, ny, -8, 8, -4, 4, 0.05, 1, 1, V, 1, psi, 2, 1));
+    RUN(orc_split2d_time(nx, ny, -8, 8, -4, 4, 0.05, 1, 1, V, 1, psi, 2, 2, 1));
+    {
+        double vr[nx * ny], out[4];
+        for (size_t i = 0; i < nx * ny; ++i) vr[i] = V[i].re;
+        RUN(orc_observables(2, nx, ny, -8, 8, -4, 4, 1.3, 0.7, 0, psi, vr, out));
+    }
     free(V);
     free(psi);
 }
@@ -120,6 +131,8 @@ static void born(void)
         RUN(orc_xsection3d(n, lo, hi, 2.0, 1, 1, p0, dp, V, 0, 0, 1, &amp));
         RUN(orc_xsection3d(n, lo, hi, 2.0, 1, 1, p0, dp, V, 0.6, 0, 0.8, NULL));
         RUN(orc_total_xsection3d(n, lo, hi, 2.0, 1, 1, p0, dp, V, 6, 5));
+        RUN(orc_born_time(3, n, lo, hi, 2.0, dir3, 1, 1, 1, 1, 0.5, 2, V, 2, 1));
+        RUN(orc_born_time(3, n, lo, hi, 2.0, dir3, 1, 1, 0, 0, 1.0, 2, V, 1, 2));
         free(V);
         free(dp);
         free(ve);
