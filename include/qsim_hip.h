@@ -262,6 +262,20 @@ int qsim_hip_split_destroy(qsim_split_job *job);
 int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps,
                                  qsim_stream_t s);
 
+/* No counterpart in the reference.  qsim_hip_split_steps_batched for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * request loops): the same sweeps and bitwise the same results, with the
+ * descriptor table in a static per-GPU device array instead of a hipMalloc'd
+ * one, written by a small kernel on `s` (no host staging).  It calls no
+ * hipMalloc/hipFree and returns once the work is enqueued.
+ * Limits: at most 512 jobs per call (QSIM_ERR_INVALID beyond); fused jobs only
+ * (power-of-two axes, periodic boundary; QSIM_ERR_UNSUPPORTED otherwise).
+ * Every call on one GPU shares that GPU's table, which keeps the last call's
+ * descriptors: calls on one stream are ordered, calls that overlap on one GPU
+ * from different streams or threads race. */
+int qsim_hip_split_steps_batched_noalloc(qsim_split_job *const *jobs, int njobs, size_t nsteps,
+                                         qsim_stream_t s);
+
 /* Reusable asynchronous batch (request loops): one launch per sweep for up
  * to `capacity` same-shape fused jobs that share method, precision and
  * launch variants.  batch_steps returns once the work is enqueued; the

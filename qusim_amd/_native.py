@@ -136,6 +136,7 @@ _SIGS = {
     "qsim_hip_split_norm2": [_VP, ctypes.POINTER(_D), _VP],
     "qsim_hip_split_destroy": [_VP],
     "qsim_hip_split_steps_batched": [ctypes.POINTER(_VP), _I, _SZ, _VP],
+    "qsim_hip_split_steps_batched_noalloc": [ctypes.POINTER(_VP), _I, _SZ, _VP],
     "qsim_hip_born_create": [ctypes.POINTER(_VP), ctypes.POINTER(BornDesc), _VP,
                              ctypes.POINTER(_D), _I],
     "qsim_hip_born_iterate": [_VP, _I, _D, ctypes.POINTER(_I), ctypes.POINTER(_D), _VP],

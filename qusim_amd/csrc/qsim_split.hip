@@ -22,6 +22,7 @@
 #include <complex>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "qsim_fft_kernels.h"
@@ -1262,6 +1263,101 @@ int qsim_hip_split_batch_destroy(qsim_split_batch *b)
     if (b->table) (void)hipFree(b->table);
     delete b;
     return QSIM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------ batched steps without allocation
+// qsim_hip_split_steps_batched_noalloc keeps the batch's descriptor table in
+// this file-scope device array instead of a hipMalloc'd one: every GPU holds
+// its own copy (kNoAllocMaxJobs * 80 B = 40 KiB) until the process ends.
+// Host code never passes `scratch` itself to a kernel or a copy (in host code
+// the name is a host-side stand-in, not a device address): it uses the
+// address hipGetSymbolAddress returns for the current device.
+namespace qsim {
+constexpr int kNoAllocMaxJobs = 512;
+static __device__ __align__(256) SplitDev scratch[kNoAllocMaxJobs];
+}  // namespace qsim
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+SplitDev *g_noalloc_table[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int noalloc_table(SplitDev **out)
+{
+    int dev = 0;
+    QSIM_HIP(hipGetDevice(&dev));
+    QSIM_CHECK(dev >= 0 && dev < kMaxDevices, "device index beyond the library's limit");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_table[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)");
+        g_noalloc_table[dev] = static_cast<SplitDev *>(p);
+    }
+    *out = g_noalloc_table[dev];
+    return QSIM_OK;
+}
+
+// Descriptors travel as kernel arguments (no host staging buffer, no
+// pageable copy): 32 per launch, 2.5 KiB of arguments.
+constexpr int kDescPerLaunch = 32;
+struct DescChunk {
+    SplitDev d[kDescPerLaunch];
+};
+
+__global__ void __launch_bounds__(64) k_put_descs(SplitDev *dst, DescChunk c, int n)
+{
+    const int i = threadIdx.x;
+    if (i < n) dst[i] = c.d[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsim_hip_split_steps_batched_noalloc(qsim_split_job *const *jobs, int njobs, size_t nsteps,
+                                         qsim_stream_t stream)
+{
+    qsim::TraceRange tr_("qsim_hip_split_steps_batched_noalloc");
+    QSIM_CHECK(jobs && njobs >= 0, "null job list");
+    QSIM_CHECK(njobs <= kNoAllocMaxJobs,
+               "qsim_hip_split_steps_batched_noalloc: more than 512 jobs in one call");
+    if (njobs == 0 || nsteps == 0) return QSIM_OK;
+    for (int i = 0; i < njobs; ++i) {
+        QSIM_CHECK(jobs[i], "null job");
+        if (!jobs[i]->fused)
+            return set_error(QSIM_ERR_UNSUPPORTED,
+                             "qsim_hip_split_steps_batched_noalloc: only fused split jobs "
+                             "(power-of-two axes, periodic boundary)");
+    }
+    const qsim_split_job *j0 = jobs[0];
+    bool same = true;
+    for (int i = 1; i < njobs && same; ++i) {
+        const qsim_split_job *ji = jobs[i];
+        same = ji->d.dim == j0->d.dim && ji->d.nx == j0->d.nx && ji->d.ny == j0->d.ny &&
+               ji->d.method == j0->d.method && ji->d.precision == j0->d.precision &&
+               ji->row_var == j0->row_var && ji->col_var == j0->col_var;
+    }
+    hipStream_t s = as_stream(stream);
+    if (!same || njobs == 1) {
+        // each job on its own device descriptor, made when it was created
+        for (int i = 0; i < njobs; ++i) QSIM_TRY(fused_steps(jobs[i], jobs[i]->dev, 1, nsteps, s));
+        return QSIM_OK;
+    }
+    SplitDev *table = nullptr;
+    QSIM_TRY(noalloc_table(&table));
+    for (int b = 0; b < njobs; b += kDescPerLaunch) {
+        DescChunk c;
+        const int m = njobs - b < kDescPerLaunch ? njobs - b : kDescPerLaunch;
+        for (int i = 0; i < m; ++i) c.d[i] = jobs[b + i]->host;
+        hipLaunchKernelGGL(k_put_descs, dim3(1), dim3(64), 0, s, table + b, c, m);
+        QSIM_HIP(hipGetLastError());
+    }
+    return fused_steps(jobs[0], table, njobs, nsteps, s);
 }
 
 }  // extern "C"
