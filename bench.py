@@ -87,33 +87,50 @@ def time_steps(job, steps, warmup):
 
 
 def kernel_times(job, steps):
+    """HIP-event time of every sweep launch of `steps` steps, by kind:
+    0 row sweep (MID), 1 column sweep, 2 1D kernel, 3 first row sweep (E_V,
+    FFT_y), 4 last row sweep (IFFT_y, E_V)."""
     from qusim_amd import _native as Nn
-    ms = (ctypes.c_double * 3)()
-    cnt = (ctypes.c_int * 3)()
+    ms = (ctypes.c_double * 5)()
+    cnt = (ctypes.c_int * 5)()
     Nn.call("qsim_hip_split_steps_timed", job, steps, ms, cnt, None)
     return list(ms), list(cnt)
 
 
+# (kind, name, algorithmic bytes per grid point by precision, launches per step)
+# FP32-mixed: work is complex64 (8 B), psi complex128 (16 B), E_V FP64 (16 B)
+SWEEP_KINDS = ((0, "row_sweep", {64: 48.0, 32: 32.0}, 1),          # work r+w, E_V r
+               (1, "col_sweep", {64: 32.0, 32: 16.0}, 1),          # work r+w
+               (3, "row_sweep_first", {64: 48.0, 32: 40.0}, 0),    # psi, E_V r; work w
+               (4, "row_sweep_last", {64: 48.0, 32: 40.0}, 0))     # work, E_V r; psi w
+
+
 def roofline(ms, cnt, npts, precision):
+    """Per-kernel average launch time and algorithmic GB/s.  The dominant
+    kernel is the one with the largest time per step (average launch time x
+    launches per step); the first and last row sweeps run once per call."""
     per = {}
-    for k, name, bpp in ((0, "row_sweep", ROW_B_PER_PT[precision]),
-                         (1, "col_sweep", COL_B_PER_PT[precision])):
+    for k, name, bpp, per_step in SWEEP_KINDS:
         if cnt[k]:
             avg_s = ms[k] / cnt[k] / 1e3
-            gbs = bpp * npts / avg_s / 1e9
-            per[name] = {"avg_us": avg_s * 1e6, "launches": cnt[k], "bytes_per_launch": bpp * npts,
-                         "achieved_GBs": gbs, "total_ms": ms[k]}
-    dom = max(per, key=lambda k: per[k]["total_ms"])
+            b = bpp[precision] * npts
+            per[name] = {"avg_us": avg_s * 1e6, "launches": cnt[k], "bytes_per_launch": b,
+                         "achieved_GBs": b / avg_s / 1e9,
+                         "frac": b / avg_s / 1e9 / HBM_PEAK_GBS,
+                         "us_per_step": avg_s * 1e6 * per_step}
+    dom = max(per, key=lambda k: per[k]["us_per_step"])
     return dom, per
 
 
-def cpu_baseline(prob, steps, threads=1):
-    from oracle import pyoracle as O
-    p = prob
-    secs = O.split2d_time(p["psi"], p["V"], p["x0"], p["x1"], p["x0"], p["x1"], p["dt"], steps,
-                          complex_v=True, threads=threads)
-    n = p["n"]
-    return steps * n * n / secs, secs
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
 
 
 def host_threads(cap=16):
@@ -299,6 +316,60 @@ def batched_rate(prob, precision, njobs, steps=40, reps=10):
             "launch_variants": [vr.value, vc.value]}
 
 
+def large_grid_section(n=4096, steps=20):
+    """The C2 construction at n^2 in complex128: psi and the E_V table are
+    256 MiB each at 4096^2, beyond the 256 MB Infinity Cache, so the sweeps
+    stream from HBM.  Autotuned variants, HIP-event timing, per-kernel bytes
+    as for the headline."""
+    from qusim_amd import _native as Nn
+    from qusim_amd.workloads import c2_problem
+    prob = c2_problem(n=n)
+    job, keep = make_job(Nn, prob, 64)
+    tuned = autotune(job, reps=5)
+    wall, gpu = time_steps(job, steps, 2)
+    ms, cnt = kernel_times(job, 4)
+    dom, per = roofline(ms, cnt, n * n, 64)
+    Nn.call("qsim_hip_split_destroy", job)
+    rate = steps * n * n / wall
+    return {"workload": "C2 construction at %dx%d, complex128, dt 0.2" % (n, n),
+            "value": rate, "unit": "grid-pt-steps/s", "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "step_frac": rate * 80.0 / 1e9 / HBM_PEAK_GBS,
+            "launch_variants": tuned, "dominant_kernel": dom, "kernels": per}
+
+
+def c1_section(threads):
+    """C1 (1D, 2048 points, dt 0.5, 1000 O2 steps) on the GPU in complex128:
+    wall time of the whole run (one fused launch), and psi, the norm and the
+    transmitted probability P(x > 0) against the oracle."""
+    from oracle import pyoracle as O
+    from qusim_amd.api import BoundaryCondition, Evolver1D, Options, SolverMethod
+    from qusim_amd import _native as Nn
+    from qusim_amd.workloads import c1_problem
+    p = c1_problem()
+    e = Evolver1D().init(p["psi"], False, p["dt"], False, p["V"], p["x0"], p["x1"], p["n"],
+                         BoundaryCondition.Period, SolverMethod.SplittingMethodO2, 1.0, 1.0,
+                         Options().Hip().Batch(p["steps"]))
+    e.step()
+    Nn.synchronize()
+    e2 = Evolver1D().init(p["psi"], False, p["dt"], False, p["V"], p["x0"], p["x1"], p["n"],
+                          BoundaryCondition.Period, SolverMethod.SplittingMethodO2, 1.0, 1.0,
+                          Options().Hip().Batch(p["steps"]))
+    Nn.synchronize()
+    t0 = time.perf_counter()
+    e2.step()
+    Nn.synchronize()
+    secs = time.perf_counter() - t0
+    ref = O.split1d(p["psi"], p["V"], p["x0"], p["x1"], p["dt"], p["steps"], method=2)
+    got = e.GetPsi()
+    pt_ref = O.norm2_range_1d(ref, p["x0"], p["x1"], 0.0, p["x1"])
+    return {"workload": "C1: 1D 2048 points on [-100,100), dt 0.5, 1000 O2 steps, complex128",
+            "gpu_seconds": secs, "value": p["steps"] * p["n"] / secs,
+            "unit": "grid-pt-steps/s",
+            "psi_rel_l2_vs_oracle": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+            "norm": e.Norm2(), "transmitted": e.Norm2(0.0, p["x1"]),
+            "transmitted_oracle": pt_ref}
+
+
 def fp32_error(prob, steps):
     """psi relative L2 error and norm drift of the FP32-mixed path against the
     complex128 path of this device after `steps` steps from the same state
@@ -320,7 +391,48 @@ def fp32_error(prob, steps):
             "steps": steps, "norm_drift_vs_fp64": (n32 - n64) / n64}
 
 
-def fp32_section(steps_err, steps_rate, fp64_plain_1024):
+def transmitted(psi, prob):
+    """P(x > 0) = sum over x_i >= 0 of |psi|^2 dx dy (x is the first axis)."""
+    n = prob["n"]
+    dx = (prob["x1"] - prob["x0"]) / n
+    x = prob["x0"] + dx * np.arange(n)
+    return float(np.sum(np.abs(psi[x >= 0.0]) ** 2) * dx * dx)
+
+
+def c3_vs_oracle(prob, steps, threads):
+    """The configuration's own check: the complex128 and the FP32-mixed GPU
+    results after `steps` steps against the FP64 oracle (the CPU restatement
+    of the reference, complex potential) from the same state: psi relative
+    L2, norm drift and the transmitted probability P(x > 0)."""
+    from oracle import pyoracle as O
+    from qusim_amd import _native as Nn
+    n = prob["n"]
+    got = {}
+    for prec in (64, 32):
+        j, keep = make_job(Nn, prob, prec)
+        Nn.call("qsim_hip_split_steps", j, steps, None)
+        got[prec] = keep[0].download().reshape(n, n)
+        Nn.call("qsim_hip_split_destroy", j)
+    t0 = time.perf_counter()
+    ref = O.split2d(prob["psi"], prob["V"], prob["x0"], prob["x1"], prob["x0"], prob["x1"],
+                    prob["dt"], steps, method=2, complex_v=True, threads=threads)
+    osecs = time.perf_counter() - t0
+    dx = (prob["x1"] - prob["x0"]) / n
+    nref = float(np.sum(np.abs(ref) ** 2) * dx * dx)
+    pref = transmitted(ref, prob)
+    out = {"steps": steps, "oracle_seconds": osecs, "oracle_threads": threads,
+           "oracle_norm": nref, "oracle_transmitted": pref}
+    for prec, tag in ((64, "complex128"), (32, "fp32_mixed")):
+        a = got[prec]
+        na = float(np.sum(np.abs(a) ** 2) * dx * dx)
+        pa = transmitted(a, prob)
+        out[tag] = {"psi_rel_l2": float(np.linalg.norm(a - ref) / np.linalg.norm(ref)),
+                    "norm_drift": (na - nref) / nref, "transmitted": pa,
+                    "transmitted_rel_err": (pa - pref) / pref}
+    return out
+
+
+def fp32_section(steps_err, steps_rate, fp64_plain_1024, oracle_threads):
     """FP32-mixed form (complex64 psi and FFTs, FP64-built factors applied as
     float-float pairs): the 2048^2 dt = 0.1 grid alone and batched, the
     1024^2 grid of the FP64 headline batched, and the psi error against the
@@ -347,6 +459,9 @@ def fp32_section(steps_err, steps_rate, fp64_plain_1024):
     b2_64 = batched_rate(p2, 64, 16)
     e3 = fp32_error(p3, steps_err)
     e2 = fp32_error(p2, steps_err)
+    log("C3 against the oracle: 2048^2, %d steps (oracle on %d threads)"
+        % (steps_err, oracle_threads))
+    c3o = c3_vs_oracle(p3, steps_err, oracle_threads)
     return {
         "workload": "FP32-mixed split-operator O2: complex64 psi and FFTs, factors built in "
                     "FP64 and applied as float-float pairs",
@@ -367,14 +482,19 @@ def fp32_section(steps_err, steps_rate, fp64_plain_1024):
                       # against the plain FP64 rate of the 1024^2 headline job
                       "speedup_single_vs_c2_fp64": rate32 / fp64_plain_1024,
                       "speedup_batched_vs_c2_fp64": b3["value"] / fp64_plain_1024,
-                      "error": e3, "dominant_kernel": dom, "kernels": per},
+                      "error": e3, "error_vs_oracle": c3o,
+                      "dominant_kernel": dom, "kernels": per},
     }
 
 
-# algorithmic bytes per point-iteration of the 3D fused passes: FP64 inner 80
-# + two middle-axis 32 + outer 32; FP32 with 128^2 y-z planes: plane END 32
-# (tmp, dpsi, complex64 V read; dpsi written) + plane START 24 + outer 16
-BORN_B_PER_PT = {64: 176.0, 32: 72.0}
+# Algorithmic bytes per point-iteration of a Born iteration: a pencil sweep
+# (x lines: read + write of one complex array) and one fused y-z plane sweep
+# reading tmp, dpsi and V and writing dpsi and tmp, with psi0 generated on the
+# fly: 16 + 40 = 56 B in complex64 (112 B in complex128).  The schedule this
+# library runs for FP32 128^2 planes spends 72 B: plane END 32 (tmp, dpsi,
+# complex64 V read; dpsi written) + plane START 24 + outer x pass 16.
+BORN_B_ALG = {64: 112.0, 32: 56.0}
+BORN_B_SCHEDULE = {64: 176.0, 32: 72.0}
 
 
 def born_section(max_iter=200, tol=1e-8, checks=(0, 7, 31)):
@@ -423,7 +543,8 @@ def born_section(max_iter=200, tol=1e-8, checks=(0, 7, 31)):
         errs.append(row)
         s64.close()
     s32.close()
-    gbs = rate * n ** 3 * BORN_B_PER_PT[32] / 1e9
+    gbs = rate * n ** 3 * BORN_B_ALG[32] / 1e9
+    roof = HBM_PEAK_GBS * 1e9 / (BORN_B_ALG[32] * n ** 3)
     return {"workload": "3D Born series 128^3 on [-32,32)^3, 32 energies k~U(1,3), Vellekoop "
                         "slow 0.5, complex64 iterates + FP64 residual, tol %g / %d iterations"
                         % (tol, max_iter),
@@ -432,7 +553,11 @@ def born_section(max_iter=200, tol=1e-8, checks=(0, 7, 31)):
             "residual_max": max(s32.residual), "converged": sum(r < tol for r in s32.residual),
             "fp64_single_energy_rate": sum(errs_i["iterations"] for errs_i in errs) / t64,
             "algorithmic_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
-            "bytes_per_point_iteration": BORN_B_PER_PT[32], "checks_vs_fp64": errs}
+            "roofline_energy_iterations_per_s": roof,
+            "bytes_per_point_iteration": BORN_B_ALG[32],
+            "schedule_bytes_per_point_iteration": BORN_B_SCHEDULE[32],
+            "schedule_GBs": rate * n ** 3 * BORN_B_SCHEDULE[32] / 1e9,
+            "checks_vs_fp64": errs}
 
 
 def serving_section(count, rate, budget_gib, spot_every=50, capacity_count=600):
@@ -464,7 +589,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="C2 steps per timed CPU run (seq and par; strong runs 5x)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--fp32-err-steps", type=int, default=1000)
@@ -560,16 +686,31 @@ def main():
             out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
             return None
 
+    log("4096^2 complex128 (beyond the Infinity Cache)")
+    g = section("grid_4096", large_grid_section)
+    if g:
+        log("4096^2 FP64: %.3e grid-pt-steps/s (%.3f ms/step), dominant %s %.2f of 8 TB/s"
+            % (g["value"], g["ms_per_step"], g["dominant_kernel"],
+               g["kernels"][g["dominant_kernel"]]["frac"]))
+    log("C1 section (1D 2048, 1000 steps)")
+    c = section("c1", lambda: c1_section(host_threads()))
+    if c:
+        log("C1: %.4f s on the GPU, psi vs oracle %.2e, P(x>0) %.10f (oracle %.10f)"
+            % (c["gpu_seconds"], c["psi_rel_l2_vs_oracle"], c["transmitted"],
+               c["transmitted_oracle"]))
     if not args.no_fp32:
         log("FP32-mixed section (2048^2 alone and batched, 1024^2 batched)")
-        f = section("fp32_mixed", lambda: fp32_section(args.fp32_err_steps, 50, value))
+        f = section("fp32_mixed", lambda: fp32_section(args.fp32_err_steps, 50, value,
+                                                       host_threads()))
         if f:
+            eo = f["grid_2048"]["error_vs_oracle"]
             log("FP32-mixed: 1024^2 x16 batched %.3e (%.2fx FP64 1024^2), 2048^2 %.3e alone, "
-                "%.3e x4 batched; psi error after %d steps %.2e (2048^2), %.2e (1024^2)"
+                "%.3e x4 batched; psi error after %d steps vs the oracle %.2e (FP32, 2048^2), "
+                "%.2e (complex128), transmitted %.6f vs %.6f"
                 % (f["value"], f["speedup_vs_fp64"], f["grid_2048"]["fp32_single"],
                    f["grid_2048"]["fp32_batched"]["value"], args.fp32_err_steps,
-                   f["grid_2048"]["error"]["psi_rel_l2_vs_fp64"],
-                   f["grid_1024"]["error"]["psi_rel_l2_vs_fp64"]))
+                   eo["fp32_mixed"]["psi_rel_l2"], eo["complex128"]["psi_rel_l2"],
+                   eo["fp32_mixed"]["transmitted"], eo["oracle_transmitted"]))
     if not args.no_born:
         log("Born 128^3 x 32 energies section")
         b = section("born_3d", born_section)
@@ -584,27 +725,65 @@ def main():
                 % (v["solves_per_s"], v["latency_p50_s"], v["latency_p99_s"]))
     if not args.no_cpu:
         nt = host_threads()
-        log("CPU baseline: %d steps of the oracle on 1 thread, %d on %d threads"
-            % (args.cpu_steps, 2 * args.cpu_steps, nt))
+        log("CPU baseline: the oracle port, seq / par / strong on up to %d threads" % nt)
         section("cpu_baseline", lambda: cpu_section(prob, args.cpu_steps, nt))
     print(json.dumps(out), flush=True)
 
 
-def cpu_section(prob, steps, nt):
-    """the oracle port timed on this host: `2 * steps` steps on nt threads
-    (the reported baseline) and `steps` on one thread beside it"""
-    rate1, secs1 = cpu_baseline(prob, steps, 1)
-    rate, secs = cpu_baseline(prob, 2 * steps, nt)
+def cpu_section(prob, steps, nt, runs=5):
+    """The FP64 oracle port of the reference's CPU path, timed on this host
+    (set-up excluded, one warm-up run, then the median of `runs`):
+    * C2 (the headline workload) in three modes: seq (1 thread, the
+      reference's CpuSeq), par (the reference's CpuPar shape: the pointwise
+      Device ops threaded over nt cores, the KISS FFT serial) and strong (the
+      FFT lines threaded too; not the reference);
+    * C4: energy-iterations/s of one 128^3 energy (Vellekoop, slow 0.5), seq
+      and par;
+    * C1: the full 2048-point, 1000-step run on one thread."""
+    from oracle import pyoracle as O
+    from qusim_amd.workloads import c1_problem, c4_energies, c4_potential
+    p = prob
+    n = p["n"]
+    a = (p["psi"], p["V"], p["x0"], p["x1"], p["x0"], p["x1"], p["dt"])
+    modes = {}
+    for name, tp, tf, st in (("seq", 1, 1, steps), ("par", nt, 1, steps),
+                             ("strong", nt, nt, 5 * steps)):
+        O.split2d_time(*a, max(1, st // 5), complex_v=True, threads=tp, fft_threads=tf)
+        secs = [O.split2d_time(*a, st, complex_v=True, threads=tp, fft_threads=tf)
+                for _ in range(runs)]
+        med = float(np.median(secs))
+        modes[name] = {"value": st * n * n / med, "unit": "grid-pt-steps/s",
+                       "threads_pointwise": tp, "threads_fft": tf, "steps_per_run": st,
+                       "runs_s": secs}
+    V4, E4 = c4_potential(n=128, L=32.0), c4_energies()[0]
+    c4 = {}
+    for name, tp, tf in (("seq", 1, 1), ("par", nt, 1)):
+        its = 2
+        secs = [O.born_time(V4, (-32.0,) * 3, (32.0,) * 3, E4, (0, 0, 1), its, True, 1, 0.5,
+                            threads=tp, fft_threads=tf) for _ in range(3)]
+        c4[name] = {"value": its / float(np.median(secs)), "unit": "energy-iterations/s",
+                    "threads_pointwise": tp, "threads_fft": tf, "iterations_per_run": its,
+                    "runs_s": secs}
+    p1 = c1_problem()
+    O.split1d(p1["psi"], p1["V"], p1["x0"], p1["x1"], p1["dt"], 10)
+    c1s = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        O.split1d(p1["psi"], p1["V"], p1["x0"], p1["x1"], p1["dt"], p1["steps"])
+        c1s.append(time.perf_counter() - t0)
+    c1m = float(np.median(c1s))
+    par = modes["par"]
     return {
-        "value": rate, "unit": "grid-pt-steps/s", "cores": nt, "kind": "port",
-        "sample": "%d O2 steps of the same 1024^2 complex-V workload (%.1f s wall on %d "
-                  "threads), FP64 oracle port of the reference's unfused op order "
-                  "(Mul, FFT2, Mul, IFFT2, Scale) with KISS-style mixed-radix FFTs; "
-                  "FFT lines and pointwise ops threaded, i.e. stronger than the "
-                  "reference's CpuPar, which threads only the pointwise ops"
-                  % (2 * steps, secs, nt),
-        "single_thread": {"value": rate1, "cores": 1,
-                          "sample": "%d steps (%.1f s), the CpuSeq form" % (steps, secs1)}}
+        "value": par["value"], "unit": "grid-pt-steps/s", "cores": nt, "kind": "port",
+        "sample": "C2 1024^2 complex-V O2 steps, %d per run, median of %d after a warm-up; "
+                  "FP64 oracle port of the reference's unfused op order (Mul, FFT2, Mul, IFFT2, "
+                  "Scale) with KISS-style mixed-radix FFTs, in the reference's CpuPar shape "
+                  "(pointwise ops on %d threads, FFT serial)" % (steps, runs, nt),
+        "cpu_model": cpu_model(), "modes": modes,
+        "c4_born_128": c4,
+        "c1_full_run": {"seconds": c1m, "value": p1["steps"] * p1["psi"].size / c1m,
+                        "unit": "grid-pt-steps/s", "threads": 1, "runs_s": c1s,
+                        "workload": "2048 points, O2, dt 0.5, 1000 steps (CpuSeq)"}}
 
 
 if __name__ == "__main__":

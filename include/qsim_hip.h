@@ -224,13 +224,17 @@ int qsim_hip_split_create(qsim_split_job **job, const qsim_split_desc *d, const 
  * space and up to date when the enqueued work completes. */
 int qsim_hip_split_steps(qsim_split_job *job, size_t nsteps, qsim_stream_t s);
 /* As qsim_hip_split_steps, timing every kernel launch with HIP events on
- * `stream`: kernel_ms[k] / kernel_launches[k] for k = 0 row sweeps, 1 column
- * sweeps, 2 the persistent 1D kernel.  Synchronous.  Fused jobs only. */
+ * `stream`: kernel_ms[k] / kernel_launches[k] (arrays of 5) for k = 0 row
+ * sweeps between steps (IFFT_y, E_V, FFT_y), 1 column sweeps, 2 the
+ * persistent 1D kernel, 3 the first row sweep of the call (E_V, FFT_y),
+ * 4 the last (IFFT_y, E_V).  Synchronous.  Fused jobs only. */
 int qsim_hip_split_steps_timed(qsim_split_job *job, size_t nsteps, double *kernel_ms,
                                int *kernel_launches, qsim_stream_t s);
 /* 2D fused jobs replay each step count's sweep sequence from a captured HIP
- * graph (off by default: on this stack a replayed graph was measured slower
- * than direct launches); enable = 1 turns it on. */
+ * graph (off by default: replay and direct launches measured the same on
+ * MI355X for a single job); enable = 1 turns it on.  At most 8 step counts
+ * are kept per job (the oldest capture is dropped first); results are bitwise
+ * those of direct launches. */
 int qsim_hip_split_use_graph(qsim_split_job *job, int enable);
 /* Launch variants of the 2D sweep kernels (radix set, lines per workgroup):
  * 0..7 for axis lengths 1024 and 2048 (6 and 7 run four radix passes with

@@ -77,20 +77,17 @@ static bool line_nt(int n, size_t line_bytes_per_launch)
 }
 
 // ---------------------------------------------------------------- 2D sweeps
-template <typename T, int N, int MODE, int V, bool NT>
-__global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
-    k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
-                const c128 *__restrict__ tw)
+// Row sweep over the L rows g0 .. g0 + L - 1 of one job (Line P, L lines of
+// P::TPL threads, LDS line stride LSTR): the body of k_row_sweep and of the
+// multi-size group kernel.
+template <typename T, typename P, int L, int LSTR, int MODE, bool NT>
+__device__ __forceinline__ void row_lines(const SplitDev &J, size_t g0, size_t nrows, int iev,
+                                          const c128 *__restrict__ tw, cx<T> *lds)
 {
-    using LP = P2Launch<T, N, false, V>;
-    using P = typename LP::Line;
-    constexpr int L = LP::L;
-    extern __shared__ unsigned char smem_raw[];
-    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-    const SplitDev &J = jobs[blockIdx.y];
+    constexpr int N = P::TPL * P::E;
     const int tid = threadIdx.x;
     const int l = tid / P::TPL, t = tid % P::TPL;
-    const size_t g = (size_t)blockIdx.x * L + l;
+    const size_t g = g0 + l;
     const bool active = g < nrows;
     const size_t base = (active ? g : 0) * N;
     const c128 *__restrict__ ev = J.ev[iev];
@@ -127,7 +124,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
                     evp[u * RA + r] = active ? ld_g(ev + base + P::template in_pos<RA>(t, u, r))
                                              : mk(0.0, 0.0);
         }
-        P::inverse(v, t, lds + l * LP::LSTR, tw);
+        P::inverse(v, t, lds + l * LSTR, tw);
 #pragma unroll
         for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
@@ -150,7 +147,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
                              conv<double>(v[u * RA + r]));
         }
     } else {
-        P::forward(v, t, lds + l * LP::LSTR, tw);
+        P::forward(v, t, lds + l * LSTR, tw);
         if (active) {
 #pragma unroll
             for (int u = 0; u < P::E / RZ; ++u)
@@ -161,21 +158,27 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
     }
 }
 
-// Column sweep over ncols columns of an nx(=N) x ncols array.
-template <typename T, int N, int V, bool NT>
-__global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
-    k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
+template <typename T, int N, int MODE, int V, bool NT>
+__global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
+    k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
                 const c128 *__restrict__ tw)
 {
-    using LP = P2Launch<T, N, true, V>;
-    using P = typename LP::Line;
-    constexpr int L = LP::L;
+    using LP = P2Launch<T, N, false, V>;
     extern __shared__ unsigned char smem_raw[];
-    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-    const SplitDev &J = jobs[blockIdx.y];
+    row_lines<T, typename LP::Line, LP::L, LP::LSTR, MODE, NT>(
+        jobs[blockIdx.y], (size_t)blockIdx.x * LP::L, nrows, iev, tw,
+        reinterpret_cast<cx<T> *>(smem_raw));
+}
+
+// Column sweep over the L columns c0 .. c0 + L - 1 of one job's nx(=N) x
+// ncols array, lanes line-fast (lane i on column i % L).
+template <typename T, typename P, int L, int LSTR, bool NT>
+__device__ __forceinline__ void col_lines(const SplitDev &J, size_t c0, size_t ncols, int ik,
+                                          const c128 *__restrict__ tw, cx<T> *lds)
+{
     const int tid = threadIdx.x;
     const int l = tid % L, t = tid / L;
-    const size_t j = xcd_tile(blockIdx.x, gridDim.x) * L + l;
+    const size_t j = c0 + l;
     const bool active = j < ncols;
     const size_t c = active ? j : 0;
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
@@ -201,7 +204,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
             for (int r = 0; r < RZ; ++r)
                 kxp[u * RZ + r] = ld_g(kx + P::template in_pos<RZ>(t, u, r));
     }
-    P::forward(v, t, lds + l * LP::LSTR, tw);
+    P::forward(v, t, lds + l * LSTR, tw);
 #pragma unroll
     for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
@@ -209,7 +212,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
             const c128 kk = PREFETCH ? kxp[u * RZ + r] : ld_g(kx + P::template in_pos<RZ>(t, u, r));
             v[u * RZ + r] = apply(v[u * RZ + r], kk * kyj);
         }
-    P::inverse(v, t, lds + l * LP::LSTR, tw);
+    P::inverse(v, t, lds + l * LSTR, tw);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / RA; ++u)
@@ -217,6 +220,18 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
             for (int r = 0; r < RA; ++r)
                 store_nt(&work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols], v[u * RA + r]);
     }
+}
+
+template <typename T, int N, int V, bool NT>
+__global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
+    k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
+                const c128 *__restrict__ tw)
+{
+    using LP = P2Launch<T, N, true, V>;
+    extern __shared__ unsigned char smem_raw[];
+    col_lines<T, typename LP::Line, LP::L, LP::LSTR, NT>(
+        jobs[blockIdx.y], xcd_tile(blockIdx.x, gridDim.x) * LP::L, ncols, ik, tw,
+        reinterpret_cast<cx<T> *>(smem_raw));
 }
 
 // ---------------------------------------------------------------- 1D job
@@ -346,6 +361,9 @@ using cd = std::complex<double>;
 
 using namespace qsim;
 
+// captured step sequences kept per job (qsim_hip_split_use_graph)
+constexpr size_t kMaxGraphs = 8;
+
 struct qsim_split_job {
     qsim_split_desc d{};
     size_t n = 0;
@@ -358,9 +376,11 @@ struct qsim_split_job {
     const void *tw_x = nullptr, *tw_y = nullptr;  // twiddles of the sweep kernels
     // launch variants of the row / column kernels (autotuned or default 0)
     int row_var = 0, col_var = 0;
-    // replay captured sweep sequences (keyed by step count)
+    // replay captured sweep sequences (keyed by step count, oldest first in
+    // graph_order)
     bool use_graph = false;
     std::map<size_t, hipGraphExec_t> graphs;
+    std::vector<size_t> graph_order;
     // phase program
     int ev_start = 0, ev_end = 0, nph = 0;
     int ph_k[3] = {0, 0, 0}, ph_e[3] = {0, 0, 0};
@@ -596,9 +616,10 @@ bool fused_len_ok(size_t n, int prec, int dim)
 }
 
 // Optional per-kernel timing: HIP events around every sweep launch.
+enum { TK_ROW = 0, TK_COL = 1, TK_1D = 2, TK_ROW_FIRST = 3, TK_ROW_LAST = 4, TK_KINDS = 5 };
 struct SweepTimer {
     std::vector<hipEvent_t> ev;  // pairs (start, stop)
-    std::vector<int> kind;       // 0 row (incl. first/last), 1 column, 2 1D
+    std::vector<int> kind;       // a TK_* per pair
     int mark(int k, hipStream_t s)
     {
         hipEvent_t e;
@@ -637,20 +658,23 @@ int fused_steps(qsim_split_job *j, const SplitDev *jobs, int njobs, size_t nstep
             p.k[i] = j->ph_k[i];
             p.e[i] = j->ph_e[i];
         }
-        TIMED(tm, 2, run_1d(prec, j->d.nx, jobs, njobs, p, nsteps, j->tw_x, s));
+        TIMED(tm, TK_1D, run_1d(prec, j->d.nx, jobs, njobs, p, nsteps, j->tw_x, s));
         return QSIM_OK;
     }
     const size_t nx = j->d.nx, ny = j->d.ny;
     const int rv = j->row_var, cv = j->col_var;
-    TIMED(tm, 0, row_sweep(prec, ny, SW_FIRST, rv, jobs, njobs, nx, j->ev_start, j->tw_y, s));
+    TIMED(tm, TK_ROW_FIRST,
+          row_sweep(prec, ny, SW_FIRST, rv, jobs, njobs, nx, j->ev_start, j->tw_y, s));
     for (size_t st = 0; st < nsteps; ++st) {
         for (int ph = 0; ph < j->nph; ++ph) {
-            TIMED(tm, 1, col_sweep(prec, nx, cv, jobs, njobs, ny, j->ph_k[ph], j->tw_x, s));
+            TIMED(tm, TK_COL, col_sweep(prec, nx, cv, jobs, njobs, ny, j->ph_k[ph], j->tw_x, s));
             const bool last = (st + 1 == nsteps) && (ph + 1 == j->nph);
             if (last)
-                TIMED(tm, 0, row_sweep(prec, ny, SW_LAST, rv, jobs, njobs, nx, j->ev_end, j->tw_y, s));
+                TIMED(tm, TK_ROW_LAST,
+                      row_sweep(prec, ny, SW_LAST, rv, jobs, njobs, nx, j->ev_end, j->tw_y, s));
             else
-                TIMED(tm, 0, row_sweep(prec, ny, SW_MID, rv, jobs, njobs, nx, j->ph_e[ph], j->tw_y, s));
+                TIMED(tm, TK_ROW,
+                      row_sweep(prec, ny, SW_MID, rv, jobs, njobs, nx, j->ph_e[ph], j->tw_y, s));
         }
     }
     return QSIM_OK;
@@ -721,10 +745,16 @@ int unfused_steps(qsim_split_job *j, size_t nsteps, hipStream_t s)
     return QSIM_OK;
 }
 
+// Captured graphs may still be replaying on some stream: wait for the device
+// before destroying them (only when the job has any, so jobs that never use
+// graphs never wait here).
 void drop_graphs(qsim_split_job *j)
 {
+    if (j->graphs.empty()) return;
+    (void)hipDeviceSynchronize();
     for (auto &kv : j->graphs) (void)hipGraphExecDestroy(kv.second);
     j->graphs.clear();
+    j->graph_order.clear();
 }
 
 void destroy(qsim_split_job *j)
@@ -924,8 +954,17 @@ int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
     if (!j->fused) return unfused_steps(j, nsteps, s);
     if (!j->use_graph || j->d.dim != 2) return fused_steps(j, j->dev, 1, nsteps, s);
     // the sweep sequence of `nsteps` steps is captured once into a HIP graph
-    // and replayed (one launch instead of 2*nsteps+1)
+    // and replayed (one launch instead of 2*nsteps+1); at most kMaxGraphs
+    // step counts are kept per job, the oldest capture is dropped first
     auto it = j->graphs.find(nsteps);
+    if (it == j->graphs.end() && j->graphs.size() >= kMaxGraphs) {
+        // a replay of the graph being dropped may still be in flight
+        QSIM_HIP(hipDeviceSynchronize());
+        auto old = j->graphs.find(j->graph_order.front());
+        (void)hipGraphExecDestroy(old->second);
+        j->graphs.erase(old);
+        j->graph_order.erase(j->graph_order.begin());
+    }
     if (it == j->graphs.end()) {
         hipStream_t cs;
         QSIM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -948,6 +987,7 @@ int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
         (void)hipStreamDestroy(cs);
         if (rc != QSIM_OK) return rc;
         it = j->graphs.emplace(nsteps, ex).first;
+        j->graph_order.push_back(nsteps);
     }
     QSIM_HIP(hipGraphLaunch(it->second, s));
     return QSIM_OK;
@@ -959,7 +999,7 @@ int qsim_hip_split_steps_timed(qsim_split_job *j, size_t nsteps, double *kernel_
     QSIM_CHECK(j && kernel_ms && kernel_launches, "null argument");
     if (!j->fused)
         return set_error(QSIM_ERR_UNSUPPORTED, "per-kernel timing needs a fused job");
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < TK_KINDS; ++k) {
         kernel_ms[k] = 0;
         kernel_launches[k] = 0;
     }
