@@ -67,7 +67,9 @@ def autotune(job, reps=20):
     ru, cu = ctypes.c_double(), ctypes.c_double()
     Nn.call("qsim_hip_split_autotune", job, reps, ctypes.byref(vr), ctypes.byref(vc),
             ctypes.byref(ru), ctypes.byref(cu), None)
-    return {"row_variant": vr.value, "col_variant": vc.value,
+    xs = ctypes.c_int()
+    Nn.call("qsim_hip_split_get_schedule", job, None, None, ctypes.byref(xs))
+    return {"row_variant": vr.value, "col_variant": vc.value, "xsplit": xs.value,
             "pair_us": cu.value}  # one alternating row + column sweep pair
 
 

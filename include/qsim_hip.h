@@ -242,10 +242,24 @@ int qsim_hip_split_use_graph(qsim_split_job *job, int enable);
  * pair measured fastest on MI355X for its lengths (FP64 1024: 4/4, FP32
  * 1024: 7/7, FP64 2048: 7/5, FP32 2048: 7/3). */
 int qsim_hip_split_set_variants(qsim_split_job *job, int row_variant, int col_variant);
-/* Autotuner: times every row variant, then every column variant, each in reps
- * alternating (row, column) sweep pairs on a copy of the state, keeps the
- * fastest on the job and reports it.  *row_us / *col_us: microseconds per
- * pair at the end of the row and of the column stage.  Synchronous. */
+/* x-split factor R of a fused 2D job (1, 2 or 4): the x transform is taken
+ * as R-point transforms across the rows {i2 + k nx/R} inside the row sweep
+ * (lane exchanges, with the four-step twiddles) and transforms of nx / R
+ * points in the column sweep, which then has R * ny lines.  The same
+ * mathematics in a different order: results agree with R = 1 to rounding.
+ * Setting R resets the column variant to the default of the new column
+ * length.  QSIM_ERR_UNSUPPORTED when nx / R has no fused kernel or a group
+ * of R rows does not fit one workgroup. */
+int qsim_hip_split_set_xsplit(qsim_split_job *job, int xsplit);
+/* The job's current launch schedule (after autotuning, for example). */
+int qsim_hip_split_get_schedule(const qsim_split_job *job, int *row_variant, int *col_variant,
+                                int *xsplit);
+/* Autotuner: for each x-split factor R (1, 2, 4) the job can take, times
+ * every row variant, then every column variant, each in reps alternating
+ * (row, column) sweep pairs on a copy of the state; keeps the fastest
+ * (R, row, column) schedule on the job (qsim_hip_split_get_schedule reads
+ * it) and reports its variants.  *row_us / *col_us: microseconds per pair at
+ * the end of the row and of the column stage.  Synchronous. */
 int qsim_hip_split_autotune(qsim_split_job *job, int reps, int *row_variant, int *col_variant,
                             double *row_us, double *col_us, qsim_stream_t s);
 /* The same search with all njobs same-shape jobs in every launch, as
@@ -291,6 +305,34 @@ int qsim_hip_split_batch_create(qsim_split_batch **batch, int capacity);
 int qsim_hip_split_batch_steps(qsim_split_batch *batch, qsim_split_job *const *jobs, int njobs,
                                size_t nsteps, qsim_stream_t s);
 int qsim_hip_split_batch_destroy(qsim_split_batch *batch);
+
+/* ------------------------------------------------ multi-size job groups
+ * No counterpart in the reference (its solvers advance one grid at a time,
+ * QuUpdatePsi in src/SplittingUtils.h:14-64, Batch(n) steps per call at
+ * src/EvolverImpl.cpp:19).  A group advances fused 2D jobs of different grid
+ * sizes (axes powers of two from 64 to 2048; one precision and method per
+ * group; default launch schedule) together: each advance of `passes` passes
+ * is one launch per sweep for all member jobs, from a fixed grid that loops
+ * over a device work list, optionally replayed from a captured HIP graph.
+ * A job added with nsteps starts at the next advance and runs the sweep
+ * sequence of qsim_hip_split_steps(job, nsteps) -- bitwise the same result --
+ * spread over as many advances as it needs (nph passes per step: 1 for O2, 3
+ * for O4, plus one closing row pass); jobs join and leave between advances
+ * without a new capture.  psi of a job is final once status reports it
+ * finished and the enqueued work has completed; a job removed before it
+ * finishes is left mid-step (its psi is not meaningful).  One stream per
+ * group: advances read tables that the next add/remove rewrites in stream
+ * order. */
+typedef struct qsim_split_group qsim_split_group;
+int qsim_hip_split_group_create(qsim_split_group **group, int capacity, int precision,
+                                int method);
+int qsim_hip_split_group_add(qsim_split_group *group, qsim_split_job *job, size_t nsteps);
+int qsim_hip_split_group_remove(qsim_split_group *group, qsim_split_job *job);
+int qsim_hip_split_group_advance(qsim_split_group *group, int passes, int use_graph,
+                                 qsim_stream_t s);
+int qsim_hip_split_group_status(const qsim_split_group *group, const qsim_split_job *job,
+                                int *finished, long long *passes_left);
+int qsim_hip_split_group_destroy(qsim_split_group *group);
 
 /* ------------------------------------------------------ fused Born job
  * QuPerturbation2DImpl/3DImpl (src/Perturbation2DImpl.cpp,

@@ -137,6 +137,16 @@ _SIGS = {
     "qsim_hip_split_destroy": [_VP],
     "qsim_hip_split_steps_batched": [ctypes.POINTER(_VP), _I, _SZ, _VP],
     "qsim_hip_split_steps_batched_noalloc": [ctypes.POINTER(_VP), _I, _SZ, _VP],
+    "qsim_hip_split_set_xsplit": [_VP, _I],
+    "qsim_hip_split_group_create": [ctypes.POINTER(_VP), _I, _I, _I],
+    "qsim_hip_split_group_add": [_VP, _VP, _SZ],
+    "qsim_hip_split_group_remove": [_VP, _VP],
+    "qsim_hip_split_group_advance": [_VP, _I, _I, _VP],
+    "qsim_hip_split_group_status": [_VP, _VP, ctypes.POINTER(_I),
+                                    ctypes.POINTER(ctypes.c_longlong)],
+    "qsim_hip_split_group_destroy": [_VP],
+    "qsim_hip_split_get_schedule": [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                    ctypes.POINTER(_I)],
     "qsim_hip_born_create": [ctypes.POINTER(_VP), ctypes.POINTER(BornDesc), _VP,
                              ctypes.POINTER(_D), _I],
     "qsim_hip_born_iterate": [_VP, _I, _D, ctypes.POINTER(_I), ctypes.POINTER(_D), _VP],

@@ -158,6 +158,185 @@ __device__ __forceinline__ void row_lines(const SplitDev &J, size_t g0, size_t n
     }
 }
 
+// ---------------------------------------------- x axis split across rows
+// With nx = R * M (R = 2 or 4) the x transform is taken in two stages (the
+// four-step factorisation, x = M i1 + i2, kx = k1 + R k2):
+//   A : y[k1][i2] = w_nx^(i2 k1) * sum_i1 x[M i1 + i2] w_R^(i1 k1)
+//   B : X[k1 + R k2] = sum_i2 y[k1][i2] w_M^(i2 k2)
+// and the inverse as B' then A'.  The row sweep does A' and A on the R rows
+// {i2 + M k} of one group, which sit in R adjacent lanes (lane = R t + k), so
+// the R-point transforms are lane exchanges (DPP quad permutations, no LDS);
+// the column sweep then runs B, the kinetic factor and B' on R * ny lines of
+// length M instead of ny lines of length nx.  Between sweeps, row k1 M + i2
+// of the working array holds y[k1][i2]; at call boundaries psi is in natural
+// order.
+// DPP quad permutations: lane ^ 1 and lane ^ 2.
+constexpr int kDppXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+// One radix-2 butterfly across lanes lane and lane ^ m: the lane with the
+// bit clear takes mine + partner, the other partner - mine (one rounding each).
+template <int CTRL, typename T>
+__device__ __forceinline__ cx<T> xbfly(cx<T> a, bool upper)
+{
+    const cx<T> b = mk(dpp_mov<CTRL>(a.x), dpp_mov<CTRL>(a.y));
+    const T sg = upper ? T(-1) : T(1);  // sg * a is exact: one rounding per sum
+    return mk(b.x + sg * a.x, b.y + sg * a.y);
+}
+
+// R-point DFT across the R lanes of a group, element by element.
+// FWD: input in bit-reversed lane order, output natural (lane k holds X[k]).
+// !FWD (inverse, +i): input natural, output bit-reversed.
+template <int R, bool FWD, typename T, int E>
+__device__ __forceinline__ void xrow_dft(cx<T> (&v)[E], int k)
+{
+    if constexpr (R == 2) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = xbfly<kDppXor1>(v[e], k & 1);
+    } else {
+        static_assert(R == 4, "row groups of 2 or 4");
+        const bool b0 = k & 1, b1 = (k >> 1) & 1;
+        if constexpr (FWD) {
+            // over bit 0, then w4^(bit 0) on lanes with bit 1 set, over bit 1
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                cx<T> a = xbfly<kDppXor1>(v[e], b0);
+                if (b1 && b0) a = mk(a.y, -a.x);  // * (-i)
+                v[e] = xbfly<kDppXor2>(a, b1);
+            }
+        } else {
+            // over bit 1, then w4^-(bit 1) on lanes with bit 0 set, over bit 0
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                cx<T> a = xbfly<kDppXor2>(v[e], b1);
+                if (b0 && b1) a = mk(-a.y, a.x);  // * (+i)
+                v[e] = xbfly<kDppXor1>(a, b0);
+            }
+        }
+    }
+}
+
+// Row sweep of the x-split schedule: one group of R rows {i2 + M k} per
+// workgroup (R * P::TPL threads, lane R t + k on row k of the group).
+template <typename T, typename P, int R, int LSTR, int MODE, bool NT>
+__device__ __forceinline__ void row_group_x(const SplitDev &J, size_t i2, size_t nx, int iev,
+                                            const c128 *__restrict__ tw,
+                                            const c128 *__restrict__ twx, cx<T> *lds)
+{
+    constexpr int N = P::TPL * P::E;
+    const int tid = threadIdx.x;
+    const int k = tid % R, t = tid / R;
+    const size_t M = nx / R;
+    const int kr = R == 4 ? ((k & 1) << 1) | (k >> 1) : k;  // bit-reversed k
+    // natural-order row of lane k (holds x[M kr + i2]) and working-array row
+    // (holds y[k][i2])
+    const size_t xrow = M * kr + i2, yrow = M * k + i2;
+    const c128 *__restrict__ ev = J.ev[iev];
+    cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
+    c128 *psi = J.psi;
+    const c128 wk = ld_g(twx + i2 * k);  // w_nx^(i2 k)
+    constexpr int RA = P::RA_, RZ = P::RLAST;
+    cx<T> v[P::E];
+    if constexpr (MODE == SW_FIRST) {
+        const size_t base = xrow * N;
+#pragma unroll
+        for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+            for (int r = 0; r < RA; ++r) {
+                const int pos = P::template in_pos<RA>(t, u, r);
+                v[u * RA + r] = apply(conv<T>(ld_g(psi + base + pos)), ld_g(ev + base + pos));
+            }
+    } else {
+        const size_t ybase = yrow * N, xbase = xrow * N;
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                v[u * RZ + r] = ld_line<NT>(work + ybase + P::template in_pos<RZ>(t, u, r));
+        constexpr bool PREFETCH = P::E <= 16;
+        c128 evp[PREFETCH ? P::E : 1];
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                for (int r = 0; r < RA; ++r)
+                    evp[u * RA + r] = ld_g(ev + xbase + P::template in_pos<RA>(t, u, r));
+        }
+        // A': w_nx^-(i2 k1), then the inverse R-point transform across rows
+#pragma unroll
+        for (int e = 0; e < P::E; ++e) v[e] = apply(v[e], mk(wk.x, -wk.y));
+        xrow_dft<R, false>(v, k);
+        P::inverse(v, t, lds + k * LSTR, tw);
+#pragma unroll
+        for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+            for (int r = 0; r < RA; ++r) {
+                const int pos = P::template in_pos<RA>(t, u, r);
+                if constexpr (PREFETCH) v[u * RA + r] = apply(v[u * RA + r], evp[u * RA + r]);
+                else v[u * RA + r] = apply(v[u * RA + r], ld_g(ev + xbase + pos));
+            }
+    }
+    if constexpr (MODE == SW_LAST) {
+        const size_t base = xrow * N;
+#pragma unroll
+        for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+            for (int r = 0; r < RA; ++r)
+                store_nt(&psi[base + P::template in_pos<RA>(t, u, r)], conv<double>(v[u * RA + r]));
+    } else {
+        P::forward(v, t, lds + k * LSTR, tw);
+        // A: the forward R-point transform across rows, then w_nx^(i2 k1)
+        xrow_dft<R, true>(v, k);
+#pragma unroll
+        for (int e = 0; e < P::E; ++e) v[e] = apply(v[e], wk);
+        const size_t base = yrow * N;
+#pragma unroll
+        for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+            for (int r = 0; r < RZ; ++r)
+                store_nt(&work[base + P::template in_pos<RZ>(t, u, r)], v[u * RZ + r]);
+    }
+}
+
+// Launch shape of the x-split row sweep: R lines of Line P per workgroup,
+// lanes line-fast within each quad; the LDS line stride sends the 16 lanes
+// of a group to 16 distinct 16-byte slots (as the line-fast column tiles).
+template <typename T, int N, int V, int R>
+struct XRowLaunch {
+    using Line = typename P2Cfg<T, N, V>::Line;
+    static constexpr int THREADS = R * Line::TPL;
+    static constexpr int TARGET = 16 / R;
+    static constexpr int LSTR =
+        Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
+    static constexpr size_t LDS_BYTES = (size_t)R * LSTR * sizeof(cx<T>);
+    static constexpr bool OK = THREADS <= 1024 && LDS_BYTES <= 160 * 1024;
+};
+
+template <typename T, int N, int MODE, int V, bool NT, int R>
+__global__ void __launch_bounds__((XRowLaunch<T, N, V, R>::THREADS))
+    k_row_sweep_x(const SplitDev *__restrict__ jobs, size_t nx, int iev,
+                  const c128 *__restrict__ tw, const c128 *__restrict__ twx)
+{
+    using XL = XRowLaunch<T, N, V, R>;
+    extern __shared__ unsigned char smem_raw[];
+    row_group_x<T, typename XL::Line, R, XL::LSTR, MODE, NT>(
+        jobs[blockIdx.y], blockIdx.x, nx, iev, tw, twx, reinterpret_cast<cx<T> *>(smem_raw));
+}
+
 template <typename T, int N, int MODE, int V, bool NT>
 __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
     k_row_sweep(const SplitDev *__restrict__ jobs, size_t nrows, int iev,
@@ -170,19 +349,25 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, V>::THREADS))
         reinterpret_cast<cx<T> *>(smem_raw));
 }
 
-// Column sweep over the L columns c0 .. c0 + L - 1 of one job's nx(=N) x
-// ncols array, lanes line-fast (lane i on column i % L).
-template <typename T, typename P, int L, int LSTR, bool NT>
-__device__ __forceinline__ void col_lines(const SplitDev &J, size_t c0, size_t ncols, int ik,
+// Column sweep over the L lines g0 .. g0 + L - 1 of one job, lanes
+// line-fast (lane i on line i % L).  R = 1: line g is column g of the
+// nx(=M) x ncols array.  R > 1 (x split, nx = R M): line g is (k1, c) =
+// (g / ncols, g % ncols), the M elements at rows k1 M + i2 of column c, and
+// its output position k2 is kx = k1 + R k2 of the kinetic factor.
+template <typename T, typename P, int L, int LSTR, bool NT, int R = 1>
+__device__ __forceinline__ void col_lines(const SplitDev &J, size_t g0, size_t ncols, int ik,
                                           const c128 *__restrict__ tw, cx<T> *lds)
 {
+    constexpr int M = P::TPL * P::E;
     const int tid = threadIdx.x;
     const int l = tid % L, t = tid / L;
-    const size_t j = c0 + l;
-    const bool active = j < ncols;
-    const size_t c = active ? j : 0;
+    const size_t g = g0 + l;
+    const bool active = g < (size_t)R * ncols;
+    const size_t ga = active ? g : 0;
+    const size_t k1 = R == 1 ? 0 : ga / ncols, c = R == 1 ? ga : ga % ncols;
+    const size_t base = k1 * M * ncols + c;
     cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
-    const c128 *__restrict__ kx = J.kx[ik];
+    const c128 *__restrict__ kx = J.kx[ik] + k1;
     const c128 kyj = ld_g(J.ky[ik] + c);
     constexpr int RA = P::RA_, RZ = P::RLAST;
     cx<T> v[P::E];
@@ -191,7 +376,7 @@ __device__ __forceinline__ void col_lines(const SplitDev &J, size_t c0, size_t n
 #pragma unroll
         for (int r = 0; r < RA; ++r)
             v[u * RA + r] =
-                active ? ld_line<NT>(work + c + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
+                active ? ld_line<NT>(work + base + (size_t)P::template in_pos<RA>(t, u, r) * ncols)
                        : mk<T>(0, 0);
     // the kinetic factors depend only on the thread's output positions:
     // issue their loads with the data so they land during the forward FFT
@@ -202,14 +387,15 @@ __device__ __forceinline__ void col_lines(const SplitDev &J, size_t c0, size_t n
         for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
-                kxp[u * RZ + r] = ld_g(kx + P::template in_pos<RZ>(t, u, r));
+                kxp[u * RZ + r] = ld_g(kx + (size_t)R * P::template in_pos<RZ>(t, u, r));
     }
     P::forward(v, t, lds + l * LSTR, tw);
 #pragma unroll
     for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
         for (int r = 0; r < RZ; ++r) {
-            const c128 kk = PREFETCH ? kxp[u * RZ + r] : ld_g(kx + P::template in_pos<RZ>(t, u, r));
+            const c128 kk = PREFETCH ? kxp[u * RZ + r]
+                                     : ld_g(kx + (size_t)R * P::template in_pos<RZ>(t, u, r));
             v[u * RZ + r] = apply(v[u * RZ + r], kk * kyj);
         }
     P::inverse(v, t, lds + l * LSTR, tw);
@@ -218,18 +404,19 @@ __device__ __forceinline__ void col_lines(const SplitDev &J, size_t c0, size_t n
         for (int u = 0; u < P::E / RA; ++u)
 #pragma unroll
             for (int r = 0; r < RA; ++r)
-                store_nt(&work[c + (size_t)P::template in_pos<RA>(t, u, r) * ncols], v[u * RA + r]);
+                store_nt(&work[base + (size_t)P::template in_pos<RA>(t, u, r) * ncols],
+                         v[u * RA + r]);
     }
 }
 
-template <typename T, int N, int V, bool NT>
+template <typename T, int N, int V, bool NT, int R = 1>
 __global__ void __launch_bounds__((P2Launch<T, N, true, V>::THREADS))
     k_col_sweep(const SplitDev *__restrict__ jobs, size_t ncols, int ik,
                 const c128 *__restrict__ tw)
 {
     using LP = P2Launch<T, N, true, V>;
     extern __shared__ unsigned char smem_raw[];
-    col_lines<T, typename LP::Line, LP::L, LP::LSTR, NT>(
+    col_lines<T, typename LP::Line, LP::L, LP::LSTR, NT, R>(
         jobs[blockIdx.y], xcd_tile(blockIdx.x, gridDim.x) * LP::L, ncols, ik, tw,
         reinterpret_cast<cx<T> *>(smem_raw));
 }
@@ -376,6 +563,11 @@ struct qsim_split_job {
     const void *tw_x = nullptr, *tw_y = nullptr;  // twiddles of the sweep kernels
     // launch variants of the row / column kernels (autotuned or default 0)
     int row_var = 0, col_var = 0;
+    // x-split factor R (1, 2, 4): the column sweep runs R * ny lines of
+    // nx / R points, the row sweep the cross-row stages (tw_xm: twiddles of
+    // nx / R)
+    int xsplit = 1;
+    const void *tw_xm = nullptr;
     // replay captured sweep sequences (keyed by step count, oldest first in
     // graph_order)
     bool use_graph = false;
@@ -471,16 +663,41 @@ int launch_row(const SplitDev *jobs, int njobs, size_t nrows, int iev, const voi
     return QSIM_OK;
 }
 
-template <typename T, int N, int V>
+// x-split row sweep (R rows per workgroup); `dry` only checks that the row
+// group fits one workgroup (threads, LDS)
+template <typename T, int N, int MODE, int V, int R>
+int launch_row_x(const SplitDev *jobs, int njobs, size_t nx, int iev, const void *tw,
+                 const void *twx, hipStream_t s, bool dry)
+{
+    using XL = XRowLaunch<T, N, V, R>;
+    if constexpr (!XL::OK) {
+        return set_error(QSIM_ERR_UNSUPPORTED, "x split: the row group exceeds one workgroup");
+    } else {
+        if (dry) return QSIM_OK;
+        auto fn = k_row_sweep_x<T, N, MODE, V, false, R>;
+        if constexpr (N <= 1024)
+            if (line_nt(N, (size_t)njobs * nx * N * sizeof(cx<T>)))
+                fn = k_row_sweep_x<T, N, MODE, V, true, R>;
+        QSIM_TRY(lds_limit((const void *)fn, XL::LDS_BYTES));
+        dim3 grid((unsigned)(nx / R), (unsigned)njobs);
+        hipLaunchKernelGGL(fn, grid, dim3(XL::THREADS), XL::LDS_BYTES, s, jobs, nx, iev,
+                           (const c128 *)tw, (const c128 *)twx);
+        QSIM_HIP(hipGetLastError());
+        return QSIM_OK;
+    }
+}
+
+template <typename T, int N, int V, int R>
 int launch_col(const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
                hipStream_t s)
 {
     using LP = P2Launch<T, N, true, V>;
-    auto fn = k_col_sweep<T, N, V, false>;
+    auto fn = k_col_sweep<T, N, V, false, R>;
     if constexpr (N <= 1024)
-        if (line_nt(N, (size_t)njobs * ncols * N * sizeof(cx<T>))) fn = k_col_sweep<T, N, V, true>;
+        if (line_nt(N, (size_t)njobs * R * ncols * N * sizeof(cx<T>)))
+            fn = k_col_sweep<T, N, V, true, R>;
     QSIM_TRY(lds_limit((const void *)fn, LP::LDS_BYTES));
-    dim3 grid((unsigned)((ncols + LP::L - 1) / LP::L), (unsigned)njobs);
+    dim3 grid((unsigned)((R * ncols + LP::L - 1) / LP::L), (unsigned)njobs);
     hipLaunchKernelGGL(fn, grid, dim3(LP::THREADS), LP::LDS_BYTES, s, jobs, ncols, ik,
                        (const c128 *)tw);
     QSIM_HIP(hipGetLastError());
@@ -530,17 +747,46 @@ bool has_variants(size_t n) { return n == 1024 || n == 2048; }
 // row length ny, column variant by the column length nx).  Fixed, so jobs that
 // are never tuned (request loops, batches) still take a fast pair and two
 // runs of one job always choose the same radix plans.
-int default_variant(int prec, size_t n, bool row)
+__host__ __device__ constexpr int default_variant(int prec, size_t n, bool row)
 {
-    if (n == 1024) return prec == 64 ? 4 : 7;
-    if (n == 2048) return prec == 64 ? (row ? 7 : 5) : (row ? 7 : 3);
+    return n == 1024 ? (prec == 64 ? 4 : 7)
+                     : n == 2048 ? (prec == 64 ? (row ? 7 : 5) : (row ? 7 : 3)) : 0;
+}
+
+// The x-split row sweep depends on the radix plan only (its lines per
+// workgroup are the R rows of a group): variants with the same plan share one
+// kernel.  1024: 0/1/5 (16-16-4), 2/3 (32-32), 4 (8-8-16), 6/7 (8-8-4-4);
+// 2048: 0/1/4/5 (16-16-8), 2 (32-32-2), 3 (8-16-16), 6/7 (8-8-8-4).
+constexpr int xrow_plan(int n, int var)
+{
+    if (n == 1024) return (var == 1 || var == 5) ? 0 : (var == 3 ? 2 : (var == 7 ? 6 : var));
+    if (n == 2048) return (var == 1 || var == 4 || var == 5) ? 0 : (var == 7 ? 6 : var);
     return 0;
 }
 
-template <typename T, int NN, int MODE>
-int row_v(int var, const SplitDev *jobs, int njobs, size_t nrows, int iev, const void *tw,
-          hipStream_t s)
+template <typename T, int NN, int MODE, int R>
+int row_vx(int var, const SplitDev *jobs, int njobs, size_t nx, int iev, const void *tw,
+           const void *twx, hipStream_t s, bool dry)
 {
+    if constexpr (NN == 1024 || NN == 2048) {
+        switch (xrow_plan(NN, var)) {
+            case 2: return launch_row_x<T, NN, MODE, 2, R>(jobs, njobs, nx, iev, tw, twx, s, dry);
+            case 3: return launch_row_x<T, NN, MODE, 3, R>(jobs, njobs, nx, iev, tw, twx, s, dry);
+            case 4: return launch_row_x<T, NN, MODE, 4, R>(jobs, njobs, nx, iev, tw, twx, s, dry);
+            case 6: return launch_row_x<T, NN, MODE, 6, R>(jobs, njobs, nx, iev, tw, twx, s, dry);
+            default: break;
+        }
+    }
+    return launch_row_x<T, NN, MODE, 0, R>(jobs, njobs, nx, iev, tw, twx, s, dry);
+}
+
+template <typename T, int NN, int MODE>
+int row_v(int var, int R, const SplitDev *jobs, int njobs, size_t nrows, int iev, const void *tw,
+          const void *twx, hipStream_t s, bool dry)
+{
+    if (R == 2) return row_vx<T, NN, MODE, 2>(var, jobs, njobs, nrows, iev, tw, twx, s, dry);
+    if (R == 4) return row_vx<T, NN, MODE, 4>(var, jobs, njobs, nrows, iev, tw, twx, s, dry);
+    if (dry) return QSIM_OK;
     if constexpr (NN == 1024 || NN == 2048) {
         switch (var) {
             case 1: return launch_row<T, NN, MODE, 1>(jobs, njobs, nrows, iev, tw, s);
@@ -556,41 +802,57 @@ int row_v(int var, const SplitDev *jobs, int njobs, size_t nrows, int iev, const
     return launch_row<T, NN, MODE, 0>(jobs, njobs, nrows, iev, tw, s);
 }
 
-template <typename T, int NN>
-int col_v(int var, const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
-          hipStream_t s)
+template <typename T, int NN, int R>
+int col_vr(int var, const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
+           hipStream_t s)
 {
     if constexpr (NN == 1024 || NN == 2048) {
         switch (var) {
-            case 1: return launch_col<T, NN, 1>(jobs, njobs, ncols, ik, tw, s);
-            case 2: return launch_col<T, NN, 2>(jobs, njobs, ncols, ik, tw, s);
-            case 3: return launch_col<T, NN, 3>(jobs, njobs, ncols, ik, tw, s);
-            case 4: return launch_col<T, NN, 4>(jobs, njobs, ncols, ik, tw, s);
-            case 5: return launch_col<T, NN, 5>(jobs, njobs, ncols, ik, tw, s);
-            case 6: return launch_col<T, NN, 6>(jobs, njobs, ncols, ik, tw, s);
-            case 7: return launch_col<T, NN, 7>(jobs, njobs, ncols, ik, tw, s);
+            case 1: return launch_col<T, NN, 1, R>(jobs, njobs, ncols, ik, tw, s);
+            case 2: return launch_col<T, NN, 2, R>(jobs, njobs, ncols, ik, tw, s);
+            case 3: return launch_col<T, NN, 3, R>(jobs, njobs, ncols, ik, tw, s);
+            case 4: return launch_col<T, NN, 4, R>(jobs, njobs, ncols, ik, tw, s);
+            case 5: return launch_col<T, NN, 5, R>(jobs, njobs, ncols, ik, tw, s);
+            case 6: return launch_col<T, NN, 6, R>(jobs, njobs, ncols, ik, tw, s);
+            case 7: return launch_col<T, NN, 7, R>(jobs, njobs, ncols, ik, tw, s);
             default: break;
         }
     }
-    return launch_col<T, NN, 0>(jobs, njobs, ncols, ik, tw, s);
+    return launch_col<T, NN, 0, R>(jobs, njobs, ncols, ik, tw, s);
 }
 
-int row_sweep(int prec, size_t ny, int mode, int var, const SplitDev *jobs, int njobs,
-              size_t nrows, int iev, const void *tw, hipStream_t s)
+template <typename T, int NN>
+int col_v(int var, int R, const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
+          hipStream_t s)
 {
-#define ROWCALL(T, NN)                                                                     \
-    (mode == SW_FIRST ? row_v<T, NN, SW_FIRST>(var, jobs, njobs, nrows, iev, tw, s)         \
-                      : mode == SW_MID ? row_v<T, NN, SW_MID>(var, jobs, njobs, nrows, iev, tw, s) \
-                                       : row_v<T, NN, SW_LAST>(var, jobs, njobs, nrows, iev, tw, s))
+    if (R == 2) return col_vr<T, NN, 2>(var, jobs, njobs, ncols, ik, tw, s);
+    if (R == 4) return col_vr<T, NN, 4>(var, jobs, njobs, ncols, ik, tw, s);
+    return col_vr<T, NN, 1>(var, jobs, njobs, ncols, ik, tw, s);
+}
+
+// Row sweep of rows of length ny (nrows = nx of them); R = x-split factor
+// (twx: the length-nx twiddles of the cross-row stage).
+int row_sweep(int prec, size_t ny, int mode, int var, int R, const SplitDev *jobs, int njobs,
+              size_t nrows, int iev, const void *tw, const void *twx, hipStream_t s,
+              bool dry = false)
+{
+#define ROWCALL(T, NN)                                                                        \
+    (mode == SW_FIRST                                                                         \
+         ? row_v<T, NN, SW_FIRST>(var, R, jobs, njobs, nrows, iev, tw, twx, s, dry)           \
+         : mode == SW_MID ? row_v<T, NN, SW_MID>(var, R, jobs, njobs, nrows, iev, tw, twx, s, \
+                                                 dry)                                         \
+                          : row_v<T, NN, SW_LAST>(var, R, jobs, njobs, nrows, iev, tw, twx,   \
+                                                  s, dry))
     QSIM_SWITCH_N(prec, ny, ROWCALL)
 #undef ROWCALL
 }
 
-int col_sweep(int prec, size_t nx, int var, const SplitDev *jobs, int njobs, size_t ncols,
+// Column sweep: R * ncols lines of length m = nx / R (tw: twiddles of m).
+int col_sweep(int prec, size_t m, int var, int R, const SplitDev *jobs, int njobs, size_t ncols,
               int ik, const void *tw, hipStream_t s)
 {
-#define COLCALL(T, NN) col_v<T, NN>(var, jobs, njobs, ncols, ik, tw, s)
-    QSIM_SWITCH_N(prec, nx, COLCALL)
+#define COLCALL(T, NN) col_v<T, NN>(var, R, jobs, njobs, ncols, ik, tw, s)
+    QSIM_SWITCH_N(prec, m, COLCALL)
 #undef COLCALL
 }
 
@@ -613,6 +875,39 @@ bool fused_len_ok(size_t n, int prec, int dim)
     if (n < 64 || (n & (n - 1))) return false;
     const size_t cap = (prec == 64 ? 4096 : 8192) * (dim == 1 ? 2 : 1);
     return n <= cap;
+}
+
+// x-split factor a new job starts with.
+int default_xsplit(int prec, size_t nx)
+{
+    return (prec == 64 && nx == 4096) || (prec == 32 && nx == 8192) ? 2 : 1;
+}
+
+// x-split factors a fused 2D job can take: 1, or 2 / 4 when the column
+// lines nx / R have a fused kernel and a row group of R rows fits one
+// workgroup with row variant `rv` (threads, LDS).
+int xsplit_check(const qsim_split_job *j, int R, int rv)
+{
+    if (R == 1) return QSIM_OK;
+    if (R != 2 && R != 4) return set_error(QSIM_ERR_INVALID, "x split factor must be 1, 2 or 4");
+    if (!j->fused || j->d.dim != 2)
+        return set_error(QSIM_ERR_UNSUPPORTED, "x split needs a fused 2D job");
+    const int prec = j->d.precision;
+    if (j->d.nx % R || !fused_len_ok(j->d.nx / R, prec, 2))
+        return set_error(QSIM_ERR_UNSUPPORTED, "x split: no fused column kernel for nx / R");
+    return row_sweep(prec, j->d.ny, SW_MID, rv, R, nullptr, 0, j->d.nx, 0, nullptr, nullptr,
+                     nullptr, true);
+}
+
+// Sets the x-split factor (twiddles of nx / R for the column lines); the
+// column variant restarts from the default of the new line length.
+int set_xsplit(qsim_split_job *j, int R, int rv)
+{
+    QSIM_TRY(xsplit_check(j, R, rv));
+    if (R != 1) QSIM_TRY(twiddle_table(j->d.nx / R, j->d.precision, &j->tw_xm));
+    if (R != j->xsplit) j->col_var = default_variant(j->d.precision, j->d.nx / R, false);
+    j->xsplit = R;
+    return QSIM_OK;
 }
 
 // Optional per-kernel timing: HIP events around every sweep launch.
@@ -662,19 +957,21 @@ int fused_steps(qsim_split_job *j, const SplitDev *jobs, int njobs, size_t nstep
         return QSIM_OK;
     }
     const size_t nx = j->d.nx, ny = j->d.ny;
-    const int rv = j->row_var, cv = j->col_var;
+    const int rv = j->row_var, cv = j->col_var, R = j->xsplit;
+    const void *twc = R == 1 ? j->tw_x : j->tw_xm;
     TIMED(tm, TK_ROW_FIRST,
-          row_sweep(prec, ny, SW_FIRST, rv, jobs, njobs, nx, j->ev_start, j->tw_y, s));
+          row_sweep(prec, ny, SW_FIRST, rv, R, jobs, njobs, nx, j->ev_start, j->tw_y, j->tw_x, s));
     for (size_t st = 0; st < nsteps; ++st) {
         for (int ph = 0; ph < j->nph; ++ph) {
-            TIMED(tm, TK_COL, col_sweep(prec, nx, cv, jobs, njobs, ny, j->ph_k[ph], j->tw_x, s));
+            TIMED(tm, TK_COL,
+                  col_sweep(prec, nx / R, cv, R, jobs, njobs, ny, j->ph_k[ph], twc, s));
             const bool last = (st + 1 == nsteps) && (ph + 1 == j->nph);
             if (last)
-                TIMED(tm, TK_ROW_LAST,
-                      row_sweep(prec, ny, SW_LAST, rv, jobs, njobs, nx, j->ev_end, j->tw_y, s));
+                TIMED(tm, TK_ROW_LAST, row_sweep(prec, ny, SW_LAST, rv, R, jobs, njobs, nx,
+                                                 j->ev_end, j->tw_y, j->tw_x, s));
             else
-                TIMED(tm, TK_ROW,
-                      row_sweep(prec, ny, SW_MID, rv, jobs, njobs, nx, j->ph_e[ph], j->tw_y, s));
+                TIMED(tm, TK_ROW, row_sweep(prec, ny, SW_MID, rv, R, jobs, njobs, nx,
+                                            j->ph_e[ph], j->tw_y, j->tw_x, s));
         }
     }
     return QSIM_OK;
@@ -887,7 +1184,16 @@ int create(qsim_split_job *j, const double *Vre, const double *Vim)
         }
         h.work = j->work;
         QSIM_TRY(twiddle_table(d.nx, d.precision, &j->tw_x));
-        if (d.dim == 2) QSIM_TRY(twiddle_table(d.ny, d.precision, &j->tw_y));
+        if (d.dim == 2) {
+            QSIM_TRY(twiddle_table(d.ny, d.precision, &j->tw_y));
+            // columns too long for a tile of several lines in LDS (FP64 4096,
+            // FP32 8192) start x-split in two: measured on MI355X at 4096^2
+            // FP64, the column sweep went from 504 to 178 us and the step
+            // from 699 to 479 us
+            const int R = default_xsplit(d.precision, d.nx);
+            if (R != 1 && xsplit_check(j, R, j->row_var) == QSIM_OK)
+                QSIM_TRY(set_xsplit(j, R, j->row_var));
+        }
     } else {
         size_t dims[2] = {d.nx, d.ny};
         size_t tn = j->n;
@@ -1031,9 +1337,29 @@ int qsim_hip_split_set_variants(qsim_split_job *j, int row_variant, int col_vari
     QSIM_CHECK(row_variant >= 0 && row_variant < kVariants && col_variant >= 0 &&
                    col_variant < kVariants,
                "launch variant out of range");
-    j->row_var = has_variants(j->d.ny) ? row_variant : 0;
-    j->col_var = has_variants(j->d.nx) ? col_variant : 0;
+    const int rv = has_variants(j->d.ny) ? row_variant : 0;
+    if (j->fused && j->d.dim == 2) QSIM_TRY(xsplit_check(j, j->xsplit, rv));
+    j->row_var = rv;
+    j->col_var = has_variants(j->d.nx / j->xsplit) ? col_variant : 0;
     drop_graphs(j);
+    return QSIM_OK;
+}
+
+int qsim_hip_split_set_xsplit(qsim_split_job *j, int xsplit)
+{
+    QSIM_CHECK(j, "null job");
+    QSIM_TRY(set_xsplit(j, xsplit, j->row_var));
+    drop_graphs(j);
+    return QSIM_OK;
+}
+
+int qsim_hip_split_get_schedule(const qsim_split_job *j, int *row_variant, int *col_variant,
+                                int *xsplit)
+{
+    QSIM_CHECK(j, "null job");
+    if (row_variant) *row_variant = j->row_var;
+    if (col_variant) *col_variant = j->col_var;
+    if (xsplit) *xsplit = j->xsplit;
     return QSIM_OK;
 }
 
@@ -1081,39 +1407,60 @@ int autotune_jobs(qsim_split_job *const *jobs, int njobs, int reps, int *row_var
     int rc = QSIM_OK;
     // The sweeps run alternately (each reads what the other wrote, from
     // other XCDs' caches), so every candidate is timed inside alternating
-    // (row, column) pairs: first the row variant with the column variant
-    // fixed, then the column variant with the chosen row variant.
-    int var[2] = {j->row_var, j->col_var};
-    double best[2] = {1e30, 1e30};
-    auto pair = [&](int vr_, int vc_) {
-        int r2 =
-            row_sweep(prec, j->d.ny, SW_MID, vr_, jt, njobs, j->d.nx, j->ph_e[0], j->tw_y, s);
-        if (r2 == QSIM_OK)
-            r2 = col_sweep(prec, j->d.nx, vc_, jt, njobs, j->d.ny, j->ph_k[0], j->tw_x, s);
-        return r2;
-    };
-    for (int which = 0; which < 2 && rc == QSIM_OK; ++which) {
-        const size_t nlen = which == 0 ? j->d.ny : j->d.nx;
-        const int nv = has_variants(nlen) ? kVariants : 1;
-        int chosen = var[which];
-        for (int v = 0; v < nv && rc == QSIM_OK; ++v) {
-            const int vr_ = which == 0 ? v : var[0], vc_ = which == 1 ? v : var[1];
-            for (int w = 0; w < 2 && rc == QSIM_OK; ++w) rc = pair(vr_, vc_);
-            if (rc != QSIM_OK) break;
-            (void)hipEventRecord(a, s);
-            for (int r = 0; r < reps && rc == QSIM_OK; ++r) rc = pair(vr_, vc_);
-            (void)hipEventRecord(b, s);
-            if (rc != QSIM_OK) break;
-            (void)hipEventSynchronize(b);
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, a, b);
-            const double us = 1e3 * ms / reps;
-            if (us < best[which]) {
-                best[which] = us;
-                chosen = v;
+    // (row, column) pairs.  For each x-split factor R the job can take: the
+    // row variant with the column variant fixed, then the column variant
+    // with the chosen row variant; the fastest (R, row, column) wins.
+    int bestR = j->xsplit, bestv[2] = {j->row_var, j->col_var};
+    double bestpair = 1e30, bestus[2] = {0, 0};
+    for (int R = 1; R <= 4 && rc == QSIM_OK; R *= 2) {
+        const size_t m = j->d.nx / R;
+        if (xsplit_check(j, R, 0) != QSIM_OK && xsplit_check(j, R, default_variant(prec, j->d.ny, true)) != QSIM_OK)
+            continue;
+        const void *twc = j->tw_x;
+        if (R != 1 && twiddle_table(m, prec, &twc) != QSIM_OK) continue;
+        int var[2] = {R == j->xsplit ? j->row_var : default_variant(prec, j->d.ny, true),
+                      R == j->xsplit ? j->col_var : default_variant(prec, m, false)};
+        if (xsplit_check(j, R, var[0]) != QSIM_OK) var[0] = 0;
+        double best[2] = {1e30, 1e30};
+        auto pair = [&](int vr_, int vc_) {
+            int r2 = row_sweep(prec, j->d.ny, SW_MID, vr_, R, jt, njobs, j->d.nx, j->ph_e[0],
+                               j->tw_y, j->tw_x, s);
+            if (r2 == QSIM_OK)
+                r2 = col_sweep(prec, m, vc_, R, jt, njobs, j->d.ny, j->ph_k[0], twc, s);
+            return r2;
+        };
+        for (int which = 0; which < 2 && rc == QSIM_OK; ++which) {
+            const size_t nlen = which == 0 ? j->d.ny : m;
+            const int nv = has_variants(nlen) ? kVariants : 1;
+            int chosen = var[which];
+            for (int v = 0; v < nv && rc == QSIM_OK; ++v) {
+                const int vr_ = which == 0 ? v : var[0], vc_ = which == 1 ? v : var[1];
+                if (which == 0 && xsplit_check(j, R, vr_) != QSIM_OK) continue;
+                for (int w = 0; w < 2 && rc == QSIM_OK; ++w) rc = pair(vr_, vc_);
+                if (rc != QSIM_OK) break;
+                (void)hipEventRecord(a, s);
+                for (int r = 0; r < reps && rc == QSIM_OK; ++r) rc = pair(vr_, vc_);
+                (void)hipEventRecord(b, s);
+                if (rc != QSIM_OK) break;
+                (void)hipEventSynchronize(b);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, a, b);
+                const double us = 1e3 * ms / reps;
+                if (us < best[which]) {
+                    best[which] = us;
+                    chosen = v;
+                }
             }
+            var[which] = chosen;
         }
-        var[which] = chosen;
+        if (rc == QSIM_OK && best[1] < bestpair) {
+            bestpair = best[1];
+            bestR = R;
+            bestv[0] = var[0];
+            bestv[1] = var[1];
+            bestus[0] = best[0];
+            bestus[1] = best[1];
+        }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
@@ -1121,14 +1468,15 @@ int autotune_jobs(qsim_split_job *const *jobs, int njobs, int reps, int *row_var
     release();
     if (rc != QSIM_OK) return rc;
     for (int i = 0; i < njobs; ++i) {
-        jobs[i]->row_var = var[0];
-        jobs[i]->col_var = var[1];
+        QSIM_TRY(set_xsplit(jobs[i], bestR, bestv[0]));
+        jobs[i]->row_var = bestv[0];
+        jobs[i]->col_var = bestv[1];
         drop_graphs(jobs[i]);
     }
-    if (row_variant) *row_variant = var[0];
-    if (col_variant) *col_variant = var[1];
-    if (row_us) *row_us = best[0];
-    if (col_us) *col_us = best[1];
+    if (row_variant) *row_variant = bestv[0];
+    if (col_variant) *col_variant = bestv[1];
+    if (row_us) *row_us = bestus[0];
+    if (col_us) *col_us = bestus[1];
     return QSIM_OK;
 }
 
@@ -1209,7 +1557,8 @@ int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps
         QSIM_CHECK(ji, "null job");
         same = ji->fused && ji->d.dim == j0->d.dim && ji->d.nx == j0->d.nx &&
                ji->d.ny == j0->d.ny && ji->d.method == j0->d.method &&
-               ji->d.precision == j0->d.precision;
+               ji->d.precision == j0->d.precision && ji->row_var == j0->row_var &&
+               ji->col_var == j0->col_var && ji->xsplit == j0->xsplit;
     }
     hipStream_t s = as_stream(stream);
     if (!same || njobs == 1) {
@@ -1280,8 +1629,8 @@ int qsim_hip_split_batch_steps(qsim_split_batch *b, qsim_split_job *const *jobs,
         QSIM_CHECK(ji && ji->fused && ji->d.dim == j0->d.dim && ji->d.nx == j0->d.nx &&
                        ji->d.ny == j0->d.ny && ji->d.method == j0->d.method &&
                        ji->d.precision == j0->d.precision && ji->row_var == j0->row_var &&
-                       ji->col_var == j0->col_var,
-                   "batch: jobs differ in shape, method, precision or launch variants");
+                       ji->col_var == j0->col_var && ji->xsplit == j0->xsplit,
+                   "batch: jobs differ in shape, method, precision or launch schedule");
     }
     hipStream_t s = as_stream(stream);
     // the pinned staging area may still feed the previous upload
@@ -1380,7 +1729,8 @@ int qsim_hip_split_steps_batched_noalloc(qsim_split_job *const *jobs, int njobs,
         const qsim_split_job *ji = jobs[i];
         same = ji->d.dim == j0->d.dim && ji->d.nx == j0->d.nx && ji->d.ny == j0->d.ny &&
                ji->d.method == j0->d.method && ji->d.precision == j0->d.precision &&
-               ji->row_var == j0->row_var && ji->col_var == j0->col_var;
+               ji->row_var == j0->row_var && ji->col_var == j0->col_var &&
+               ji->xsplit == j0->xsplit;
     }
     hipStream_t s = as_stream(stream);
     if (!same || njobs == 1) {
@@ -1398,6 +1748,447 @@ int qsim_hip_split_steps_batched_noalloc(qsim_split_job *const *jobs, int njobs,
         QSIM_HIP(hipGetLastError());
     }
     return fused_steps(jobs[0], table, njobs, nsteps, s);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ multi-size job groups
+// A group advances fused 2D split jobs of different grid sizes together: one
+// launch per sweep for all of them, from a fixed grid of workgroups that
+// loops over a device-side list of (job, tile) items.  Each job keeps its own
+// step program: the row pass of global index P runs job j's FIRST sweep when
+// P == first_pass(j), its LAST when P == first_pass(j) + nph * nsteps(j),
+// a MID sweep in between and nothing otherwise; the column pass P runs while
+// first_pass(j) <= P < first_pass(j) + nph * nsteps(j).  So every job sees
+// exactly the sweep sequence of qsim_hip_split_steps(job, nsteps), in the
+// same radix plans (its default launch variants), and its result is the
+// same bit for bit, whatever else shares the launches.  The pass index base
+// lives in device memory and a final kernel of each advance adds the number
+// of passes: a captured advance stays valid while jobs join and leave.
+namespace qsim {
+
+constexpr int kGroupThreads = 256;
+
+struct GroupJob {
+    SplitDev d;
+    const c128 *twx, *twy;
+    unsigned nx, ny;
+    int nph, ev_start, ev_end;
+    int ph_k[3], ph_e[3];
+    long long first_pass, nsteps;
+};
+
+struct GroupItem {
+    int job;        // -1: padding
+    unsigned line0; // first row (row pass) or column (column pass) of the tile
+};
+
+// One length's launch shape in the group kernels: the job's default radix
+// plan, kGroupThreads / TPL lines per workgroup, the standalone LDS strides.
+template <typename T, int N, bool LF>
+struct GroupLaunch {
+    static constexpr int V = default_variant(sizeof(T) == 8 ? 64 : 32, N, !LF);
+    using Line = typename P2Cfg<T, N, V>::Line;
+    static_assert(Line::TPL <= kGroupThreads, "one line per group workgroup at most");
+    static constexpr int L = kGroupThreads / Line::TPL;
+    static constexpr int TARGET = LF ? (L >= 16 ? 1 : 16 / L) : 1;
+    static constexpr int LSTR =
+        Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
+    static constexpr size_t LDS_BYTES = (size_t)L * LSTR * sizeof(cx<T>);
+};
+
+__host__ __device__ constexpr size_t zmax(size_t a, size_t b) { return a > b ? a : b; }
+template <typename T, int N>
+__host__ __device__ constexpr size_t group_lds_n()
+{
+    return zmax(GroupLaunch<T, N, false>::LDS_BYTES, GroupLaunch<T, N, true>::LDS_BYTES);
+}
+// dynamic LDS of the group kernels: the largest over the lengths they take
+template <typename T>
+__host__ __device__ constexpr size_t group_lds()
+{
+    return zmax(zmax(zmax(group_lds_n<T, 64>(), group_lds_n<T, 128>()),
+                     zmax(group_lds_n<T, 256>(), group_lds_n<T, 512>())),
+                zmax(group_lds_n<T, 1024>(), group_lds_n<T, 2048>()));
+}
+
+__host__ __device__ constexpr bool group_len_ok(size_t n)
+{
+    return n >= 64 && n <= 2048 && (n & (n - 1)) == 0;
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void group_row_n(const GroupJob &G, unsigned line0, int mode, int iev,
+                                            cx<T> *lds)
+{
+    using GL = GroupLaunch<T, N, false>;
+    using P = typename GL::Line;
+    constexpr bool NT = N <= 1024;
+    if (mode == SW_FIRST)
+        row_lines<T, P, GL::L, GL::LSTR, SW_FIRST, NT>(G.d, line0, G.nx, iev, G.twy, lds);
+    else if (mode == SW_MID)
+        row_lines<T, P, GL::L, GL::LSTR, SW_MID, NT>(G.d, line0, G.nx, iev, G.twy, lds);
+    else
+        row_lines<T, P, GL::L, GL::LSTR, SW_LAST, NT>(G.d, line0, G.nx, iev, G.twy, lds);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void group_col_n(const GroupJob &G, unsigned line0, int ik, cx<T> *lds)
+{
+    using GL = GroupLaunch<T, N, true>;
+    col_lines<T, typename GL::Line, GL::L, GL::LSTR, (N <= 1024), 1>(G.d, line0, G.ny, ik, G.twx,
+                                                                     lds);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kGroupThreads)
+    k_row_group(const GroupJob *__restrict__ jobs, const GroupItem *__restrict__ items,
+                const int *__restrict__ counts, const long long *__restrict__ basep, int r)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const long long P = *basep + r;
+    const int nitems = counts[0];
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const GroupItem it = items[w];
+        if (it.job < 0) continue;
+        const GroupJob &G = jobs[it.job];
+        const long long q = P - G.first_pass, qend = (long long)G.nph * G.nsteps;
+        if (q < 0 || q > qend) continue;  // the job is not running at this pass
+        const int mode = q == 0 ? SW_FIRST : (q == qend ? SW_LAST : SW_MID);
+        const int iev = mode == SW_FIRST ? G.ev_start
+                        : mode == SW_LAST ? G.ev_end
+                                          : G.ph_e[(int)((q - 1) % G.nph)];
+        switch (G.ny) {
+            case 64: group_row_n<T, 64>(G, it.line0, mode, iev, lds); break;
+            case 128: group_row_n<T, 128>(G, it.line0, mode, iev, lds); break;
+            case 256: group_row_n<T, 256>(G, it.line0, mode, iev, lds); break;
+            case 512: group_row_n<T, 512>(G, it.line0, mode, iev, lds); break;
+            case 1024: group_row_n<T, 1024>(G, it.line0, mode, iev, lds); break;
+            case 2048: group_row_n<T, 2048>(G, it.line0, mode, iev, lds); break;
+            default: break;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kGroupThreads)
+    k_col_group(const GroupJob *__restrict__ jobs, const GroupItem *__restrict__ items,
+                const int *__restrict__ counts, const long long *__restrict__ basep, int r)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const long long P = *basep + r;
+    const int nitems = counts[1];  // a multiple of 8 (padded)
+    const GroupItem *cols = items + counts[0];  // after the row items
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        // the grid is a multiple of 8, so workgroup b (on XCD b % 8) takes
+        // items w = b (mod 8): give each XCD a contiguous range of tiles
+        const int we = (w % 8) * (nitems / 8) + w / 8;
+        const GroupItem it = cols[we];
+        if (it.job < 0) continue;
+        const GroupJob &G = jobs[it.job];
+        const long long q = P - G.first_pass;
+        if (q < 0 || q >= (long long)G.nph * G.nsteps) continue;
+        const int ik = G.ph_k[(int)(q % G.nph)];
+        switch (G.nx) {
+            case 64: group_col_n<T, 64>(G, it.line0, ik, lds); break;
+            case 128: group_col_n<T, 128>(G, it.line0, ik, lds); break;
+            case 256: group_col_n<T, 256>(G, it.line0, ik, lds); break;
+            case 512: group_col_n<T, 512>(G, it.line0, ik, lds); break;
+            case 1024: group_col_n<T, 1024>(G, it.line0, ik, lds); break;
+            case 2048: group_col_n<T, 2048>(G, it.line0, ik, lds); break;
+            default: break;
+        }
+    }
+}
+
+// the advance's last node: the next advance starts `passes` passes later
+__global__ void __launch_bounds__(64) k_group_tick(long long *basep, int passes)
+{
+    if (threadIdx.x == 0) *basep += passes;
+}
+
+}  // namespace qsim
+
+struct qsim_split_group {
+    int prec = 64, method = 2, cap = 0;
+    std::vector<qsim_split_job *> jobs;
+    std::vector<long long> first, nsteps;
+    long long base = 0;  // host copy of the device pass index
+    int grid = 0;
+    // device tables and their pinned staging
+    GroupJob *djobs = nullptr, *hjobs = nullptr;
+    GroupItem *dItems = nullptr, *hItems = nullptr;  // row items, then column items
+    size_t item_cap = 0;
+    int *dcounts = nullptr, *hcounts = nullptr;
+    long long *dbase = nullptr;
+    bool dirty = true;
+    hipEvent_t copied = nullptr;
+    bool pending = false;
+    std::map<int, hipGraphExec_t> graphs;  // captured advances, by pass count
+};
+
+namespace {
+
+void group_drop_graphs(qsim_split_group *g)
+{
+    if (g->graphs.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (auto &kv : g->graphs) (void)hipGraphExecDestroy(kv.second);
+    g->graphs.clear();
+}
+
+template <typename T>
+int group_launch_passes(qsim_split_group *g, int passes, hipStream_t s)
+{
+    const size_t lds = group_lds<T>();
+    QSIM_TRY(lds_limit((const void *)k_row_group<T>, lds));
+    QSIM_TRY(lds_limit((const void *)k_col_group<T>, lds));
+    for (int r = 0; r < passes; ++r) {
+        hipLaunchKernelGGL(k_row_group<T>, dim3(g->grid), dim3(kGroupThreads), lds, s, g->djobs,
+                           g->dItems, g->dcounts, g->dbase, r);
+        QSIM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_col_group<T>, dim3(g->grid), dim3(kGroupThreads), lds, s, g->djobs,
+                           g->dItems, g->dcounts, g->dbase, r);
+        QSIM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_group_tick, dim3(1), dim3(64), 0, s, g->dbase, passes);
+    QSIM_HIP(hipGetLastError());
+    return QSIM_OK;
+}
+
+int group_enqueue(qsim_split_group *g, int passes, hipStream_t s)
+{
+    return g->prec == 64 ? group_launch_passes<double>(g, passes, s)
+                         : group_launch_passes<float>(g, passes, s);
+}
+
+// Rebuilds the work lists from the member jobs and uploads them (stream
+// ordered before the next advance's kernels).
+int group_upload(qsim_split_group *g, hipStream_t s)
+{
+    const int nj = (int)g->jobs.size();
+    size_t nrow = 0, ncol = 0;
+    std::vector<int> lrow(nj), lcol(nj);
+    for (int i = 0; i < nj; ++i) {
+        const qsim_split_job *j = g->jobs[i];
+        const int prec = g->prec;
+        auto lines = [prec](size_t n, bool lf) -> int {
+            switch (n) {
+#define QSIM_GL(NN)                                                                            \
+    case NN:                                                                                   \
+        return prec == 64 ? (lf ? GroupLaunch<double, NN, true>::L : GroupLaunch<double, NN, false>::L) \
+                          : (lf ? GroupLaunch<float, NN, true>::L : GroupLaunch<float, NN, false>::L);
+                QSIM_GL(64) QSIM_GL(128) QSIM_GL(256) QSIM_GL(512) QSIM_GL(1024) QSIM_GL(2048)
+#undef QSIM_GL
+                default: return 1;
+            }
+        };
+        lrow[i] = lines(j->d.ny, false);
+        lcol[i] = lines(j->d.nx, true);
+        nrow += (j->d.nx + lrow[i] - 1) / lrow[i];
+        ncol += (j->d.ny + lcol[i] - 1) / lcol[i];
+    }
+    ncol = (ncol + 7) / 8 * 8;  // column items padded to a multiple of 8
+    const size_t need = nrow + ncol;
+    if (g->pending) QSIM_HIP(hipEventSynchronize(g->copied));
+    g->pending = false;
+    if (need > g->item_cap) {
+        // new device tables: captured advances hold the old pointers
+        group_drop_graphs(g);
+        if (g->dItems) QSIM_HIP(hipFree(g->dItems));
+        if (g->hItems) QSIM_HIP(hipHostFree(g->hItems));
+        g->dItems = nullptr;
+        g->hItems = nullptr;
+        size_t cap = 1024;
+        while (cap < need) cap *= 2;
+        QSIM_HIP(hipMalloc(&g->dItems, cap * sizeof(GroupItem)));
+        QSIM_HIP(hipHostMalloc(&g->hItems, cap * sizeof(GroupItem), hipHostMallocDefault));
+        g->item_cap = cap;
+    }
+    size_t w = 0;
+    for (int i = 0; i < nj; ++i) {
+        qsim_split_job *j = g->jobs[i];
+        GroupJob &G = g->hjobs[i];
+        G.d = j->host;
+        G.twx = (const c128 *)j->tw_x;
+        G.twy = (const c128 *)j->tw_y;
+        G.nx = (unsigned)j->d.nx;
+        G.ny = (unsigned)j->d.ny;
+        G.nph = j->nph;
+        G.ev_start = j->ev_start;
+        G.ev_end = j->ev_end;
+        for (int k = 0; k < 3; ++k) {
+            G.ph_k[k] = j->ph_k[k];
+            G.ph_e[k] = j->ph_e[k];
+        }
+        G.first_pass = g->first[i];
+        G.nsteps = g->nsteps[i];
+        for (size_t r0 = 0; r0 < j->d.nx; r0 += lrow[i]) g->hItems[w++] = GroupItem{i, (unsigned)r0};
+    }
+    for (int i = 0; i < nj; ++i)
+        for (size_t c0 = 0; c0 < g->jobs[i]->d.ny; c0 += lcol[i])
+            g->hItems[w++] = GroupItem{i, (unsigned)c0};
+    while (w < need) g->hItems[w++] = GroupItem{-1, 0};
+    g->hcounts[0] = (int)nrow;
+    g->hcounts[1] = (int)ncol;
+    if (nj)
+        QSIM_HIP(hipMemcpyAsync(g->djobs, g->hjobs, sizeof(GroupJob) * nj, hipMemcpyHostToDevice,
+                                s));
+    if (need)
+        QSIM_HIP(hipMemcpyAsync(g->dItems, g->hItems, sizeof(GroupItem) * need,
+                                hipMemcpyHostToDevice, s));
+    QSIM_HIP(hipMemcpyAsync(g->dcounts, g->hcounts, sizeof(int) * 2, hipMemcpyHostToDevice, s));
+    QSIM_HIP(hipEventRecord(g->copied, s));
+    g->pending = true;
+    g->dirty = false;
+    return QSIM_OK;
+}
+
+int group_find(const qsim_split_group *g, const qsim_split_job *j)
+{
+    for (size_t i = 0; i < g->jobs.size(); ++i)
+        if (g->jobs[i] == j) return (int)i;
+    return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsim_hip_split_group_destroy(qsim_split_group *g)
+{
+    if (!g) return QSIM_OK;
+    if (g->pending) (void)hipEventSynchronize(g->copied);
+    group_drop_graphs(g);
+    if (g->copied) (void)hipEventDestroy(g->copied);
+    if (g->djobs) (void)hipFree(g->djobs);
+    if (g->hjobs) (void)hipHostFree(g->hjobs);
+    if (g->dItems) (void)hipFree(g->dItems);
+    if (g->hItems) (void)hipHostFree(g->hItems);
+    if (g->dcounts) (void)hipFree(g->dcounts);
+    if (g->hcounts) (void)hipHostFree(g->hcounts);
+    if (g->dbase) (void)hipFree(g->dbase);
+    delete g;
+    return QSIM_OK;
+}
+
+int qsim_hip_split_group_create(qsim_split_group **out, int capacity, int precision, int method)
+{
+    QSIM_CHECK(out && capacity > 0, "group: null out-pointer or capacity < 1");
+    QSIM_CHECK(precision == 64 || precision == 32, "group: precision must be 64 or 32");
+    QSIM_CHECK(method == 2 || method == 4, "group: method must be 2 (O2) or 4 (O4)");
+    *out = nullptr;
+    qsim_split_group *g = new qsim_split_group();
+    g->prec = precision;
+    g->method = method;
+    g->cap = capacity;
+    const int cu = device_cu_count();
+    g->grid = ((cu > 0 ? 2 * cu : 512) + 7) / 8 * 8;
+    int rc = QSIM_OK;
+    const long long zero = 0;
+    if (hipMalloc(&g->djobs, sizeof(GroupJob) * capacity) != hipSuccess ||
+        hipHostMalloc(&g->hjobs, sizeof(GroupJob) * capacity, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&g->dcounts, sizeof(int) * 2) != hipSuccess ||
+        hipHostMalloc(&g->hcounts, sizeof(int) * 2, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&g->dbase, sizeof(long long)) != hipSuccess ||
+        hipEventCreateWithFlags(&g->copied, hipEventDisableTiming) != hipSuccess ||
+        hipMemcpy(g->dbase, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = set_error(QSIM_ERR_NOMEM, "group: allocation failed");
+    }
+    if (rc != QSIM_OK) {
+        qsim_hip_split_group_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return QSIM_OK;
+}
+
+int qsim_hip_split_group_add(qsim_split_group *g, qsim_split_job *j, size_t nsteps)
+{
+    QSIM_CHECK(g && j, "group: null argument");
+    QSIM_CHECK(nsteps >= 1, "group: a job needs at least one step");
+    QSIM_CHECK((int)g->jobs.size() < g->cap, "group: full");
+    QSIM_CHECK(group_find(g, j) < 0, "group: the job is already in the group");
+    QSIM_CHECK(j->fused && j->d.dim == 2 && j->d.precision == g->prec && j->d.method == g->method,
+               "group: needs a fused 2D job of the group's precision and method");
+    QSIM_CHECK(group_len_ok(j->d.nx) && group_len_ok(j->d.ny),
+               "group: grid axes must be powers of two from 64 to 2048");
+    QSIM_CHECK(j->xsplit == 1 && j->row_var == default_variant(g->prec, j->d.ny, true) &&
+                   j->col_var == default_variant(g->prec, j->d.nx, false),
+               "group: the job must keep its default launch schedule");
+    g->jobs.push_back(j);
+    g->first.push_back(g->base);
+    g->nsteps.push_back((long long)nsteps);
+    g->dirty = true;
+    return QSIM_OK;
+}
+
+int qsim_hip_split_group_remove(qsim_split_group *g, qsim_split_job *j)
+{
+    QSIM_CHECK(g && j, "group: null argument");
+    const int i = group_find(g, j);
+    QSIM_CHECK(i >= 0, "group: the job is not in the group");
+    g->jobs.erase(g->jobs.begin() + i);
+    g->first.erase(g->first.begin() + i);
+    g->nsteps.erase(g->nsteps.begin() + i);
+    g->dirty = true;
+    return QSIM_OK;
+}
+
+int qsim_hip_split_group_advance(qsim_split_group *g, int passes, int use_graph,
+                                 qsim_stream_t stream)
+{
+    qsim::TraceRange tr_("qsim_hip_split_group_advance");
+    QSIM_CHECK(g && passes >= 1, "group: null group or passes < 1");
+    hipStream_t s = as_stream(stream);
+    if (g->dirty) QSIM_TRY(group_upload(g, s));
+    if (!use_graph) {
+        QSIM_TRY(group_enqueue(g, passes, s));
+    } else {
+        auto it = g->graphs.find(passes);
+        if (it == g->graphs.end()) {
+            if (g->graphs.size() >= kMaxGraphs) group_drop_graphs(g);
+            hipStream_t cs;
+            QSIM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+            int rc = QSIM_OK;
+            if (e == hipSuccess) {
+                rc = group_enqueue(g, passes, cs);
+                hipError_t e2 = hipStreamEndCapture(cs, &gr);
+                if (rc == QSIM_OK && e2 != hipSuccess) rc = hip_error(e2, "hipStreamEndCapture");
+            } else {
+                rc = hip_error(e, "hipStreamBeginCapture");
+            }
+            hipGraphExec_t ex = nullptr;
+            if (rc == QSIM_OK) {
+                e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                if (e != hipSuccess) rc = hip_error(e, "hipGraphInstantiate");
+            }
+            if (gr) (void)hipGraphDestroy(gr);
+            (void)hipStreamDestroy(cs);
+            if (rc != QSIM_OK) return rc;
+            it = g->graphs.emplace(passes, ex).first;
+        }
+        QSIM_HIP(hipGraphLaunch(it->second, s));
+    }
+    g->base += passes;
+    return QSIM_OK;
+}
+
+int qsim_hip_split_group_status(const qsim_split_group *g, const qsim_split_job *j,
+                                int *finished, long long *passes_left)
+{
+    QSIM_CHECK(g && j, "group: null argument");
+    const int i = group_find(g, j);
+    QSIM_CHECK(i >= 0, "group: the job is not in the group");
+    const long long last = g->first[i] + (long long)j->nph * g->nsteps[i];
+    if (finished) *finished = last < g->base;
+    if (passes_left) *passes_left = last < g->base ? 0 : last + 1 - g->base;
+    return QSIM_OK;
 }
 
 }  // extern "C"

@@ -3,6 +3,8 @@ reference-order path) against the FP64 oracle (CpuSeq + KISS restatement).
 
 Bar: complex128 psi within 1e-12 relative L2 (the path's stated tolerance);
 FP32-mixed within the tolerances written beside each case."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -388,3 +390,66 @@ def test_graph_replay_is_bitwise_and_its_cache_bounded():
     N.call("qsim_hip_split_steps", a._job, 2, None)
     N.call("qsim_hip_split_steps", b._job, 2, None)
     assert np.array_equal(a.GetPsi(), b.GetPsi())
+
+
+@pytest.mark.parametrize("xsplit", [2, 4])
+@pytest.mark.parametrize("shape,method,precision",
+                         [((1024, 1024), 2, "fp64"), ((256, 512), 4, "fp64"),
+                          ((512, 128), 2, "fp64"), ((2048, 64), 2, "fp64"),
+                          ((1024, 256), 2, "mixed32"), ((256, 2048), 4, "mixed32")])
+def test_xsplit_schedule_matches_oracle(shape, method, precision, xsplit, oracle):
+    """x axis split into R-point transforms across rows (lane exchanges in the
+    row sweep) and nx/R-point column transforms: complex128 within 1e-12 of
+    the oracle, FP32-mixed within 1e-5, for every row variant the group
+    fits."""
+    from qusim_amd import _native as N
+    from qusim_amd.api import BoundaryCondition, Evolver2D, Options, SolverMethod
+    psi, V, L0, L1 = gauss2d(*shape)
+    steps = 12
+    ref = oracle.split2d(psi, V, L0[0], L0[1], L1[0], L1[1], 0.05, steps, method=method,
+                         complex_v=True, threads=8)
+    norm = np.sqrt(np.sum(np.abs(psi) ** 2) * (L0[1] - L0[0]) / shape[0]
+                   * (L1[1] - L1[0]) / shape[1])
+    variants = range(8) if shape[1] in (1024, 2048) else [0]
+    for rv in variants:
+        o = Options().Hip().Batch(steps).ComplexPotential(True)
+        if precision != "fp64":
+            o.Precision(precision)
+        e = Evolver2D().init(psi, False, 0.05, False, V, L0[0], L0[1], shape[0], L1[0], L1[1],
+                             shape[1], BoundaryCondition.Period,
+                             SolverMethod.SplittingMethodO2 if method == 2
+                             else SolverMethod.SplittingMethodO4, 1.0, 1.0, o)
+        try:
+            N.call("qsim_hip_split_set_xsplit", e._job, xsplit)
+            N.call("qsim_hip_split_set_variants", e._job, rv, 0)
+        except N.QsimError:
+            continue  # this row variant's group does not fit one workgroup
+        e.step()
+        err = rel_l2(e.GetPsi(), ref)
+        assert err < (1e-12 if precision == "fp64" else 1e-5), (rv, err)
+
+
+def test_xsplit_limits_and_autotune():
+    from qusim_amd import _native as N
+    from qusim_amd.api import BoundaryCondition, Evolver2D, Options, SolverMethod
+    psi, V, L0, L1 = gauss2d(128, 64)
+    e = Evolver2D().init(psi, True, 0.05, False, V, L0[0], L0[1], 128, L1[0], L1[1], 64,
+                         BoundaryCondition.Period, SolverMethod.SplittingMethodO2, 1.0, 1.0,
+                         Options().Hip())
+    with pytest.raises(N.QsimError, match="1, 2 or 4"):
+        N.call("qsim_hip_split_set_xsplit", e._job, 3)
+    with pytest.raises(N.QsimError, match="nx / R"):
+        N.call("qsim_hip_split_set_xsplit", e._job, 4)   # 128 / 4 = 32 < 64
+    N.call("qsim_hip_split_set_xsplit", e._job, 2)
+    a = e.GetPsi()
+    vr, vc, xs = ctypes_ints(3)
+    N.call("qsim_hip_split_autotune", e._job, 2, ctypes.byref(vr), ctypes.byref(vc), None, None,
+           None)
+    N.call("qsim_hip_split_get_schedule", e._job, ctypes.byref(vr), ctypes.byref(vc),
+           ctypes.byref(xs))
+    assert xs.value in (1, 2)
+    assert np.array_equal(e.GetPsi(), a)  # autotuning leaves the state alone
+
+
+def ctypes_ints(n):
+    return [ctypes.c_int() for _ in range(n)]
