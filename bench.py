@@ -318,6 +318,73 @@ def batched_rate(prob, precision, njobs, steps=40, reps=10):
             "launch_variants": [vr.value, vc.value]}
 
 
+def group_section(steps=120, passes=20):
+    """FP32-mixed 2D jobs of the request mix's sizes (8 x 256^2, 4 x 512^2,
+    2 x 1024^2, the C2 construction at each size) advanced `steps` steps:
+    one multi-size group (one launch per sweep for all jobs, advances of
+    `passes` passes, with and without graph replay) against the same jobs as
+    one same-size batch per size (qsim_hip_split_batch_steps), all on one
+    stream.  The first advance (which captures the graph) is not timed."""
+    from qusim_amd import _native as Nn
+    from qusim_amd.workloads import c2_problem
+    sizes = [256] * 8 + [512] * 4 + [1024] * 2
+    probs = {n: c2_problem(n=n) for n in set(sizes)}
+    pts = sum(n * n for n in sizes)
+    out = {"workload": "8 x 256^2 + 4 x 512^2 + 2 x 1024^2 FP32-mixed 2D jobs, %d steps" % steps}
+    for use_graph in (1, 0):
+        jobs = [make_job(Nn, probs[n], 32) for n in sizes]
+        g = ctypes.c_void_p()
+        Nn.call("qsim_hip_split_group_create", ctypes.byref(g), len(jobs), 32, 2)
+        g = g.value
+        for j, _ in jobs:
+            Nn.call("qsim_hip_split_group_add", g, j, steps)
+        Nn.call("qsim_hip_split_group_advance", g, passes, use_graph, None)
+        Nn.synchronize()
+        e0, e1 = Nn.Event(), Nn.Event()
+        e0.record()
+        left = steps + 1 - passes
+        nadv = 0
+        while left > 0:
+            Nn.call("qsim_hip_split_group_advance", g, passes, use_graph, None)
+            left -= passes
+            nadv += 1
+        e1.record()
+        Nn.synchronize()
+        secs = e0.elapsed_ms(e1) / 1e3
+        Nn.call("qsim_hip_split_group_destroy", g)
+        for j, _ in jobs:
+            Nn.call("qsim_hip_split_destroy", j)
+        out["group_graph" if use_graph else "group_direct"] = {
+            "value": (steps - passes) * pts / secs, "unit": "grid-pt-steps/s",
+            "advances_timed": nadv, "passes_per_advance": passes}
+    jobs = [make_job(Nn, probs[n], 32) for n in sizes]
+    by = {}
+    for (j, _), n in zip(jobs, sizes):
+        by.setdefault(n, []).append(j)
+    batches = {}
+    for n, js in by.items():
+        b = ctypes.c_void_p()
+        Nn.call("qsim_hip_split_batch_create", ctypes.byref(b), len(js))
+        batches[n] = (b.value, (ctypes.c_void_p * len(js))(*js), len(js))
+    for b, arr, k in batches.values():
+        Nn.call("qsim_hip_split_batch_steps", b, arr, k, 2, None)
+    Nn.synchronize()
+    e0, e1 = Nn.Event(), Nn.Event()
+    e0.record()
+    for b, arr, k in batches.values():
+        Nn.call("qsim_hip_split_batch_steps", b, arr, k, steps, None)
+    e1.record()
+    Nn.synchronize()
+    secs = e0.elapsed_ms(e1) / 1e3
+    for b, _, _ in batches.values():
+        Nn.call("qsim_hip_split_batch_destroy", b)
+    for j, _ in jobs:
+        Nn.call("qsim_hip_split_destroy", j)
+    out["per_size_batches"] = {"value": steps * pts / secs, "unit": "grid-pt-steps/s"}
+    out["group_vs_batches"] = out["group_graph"]["value"] / out["per_size_batches"]["value"]
+    return out
+
+
 def large_grid_section(n=4096, steps=20):
     """The C2 construction at n^2 in complex128: psi and the E_V table are
     256 MiB each at 4096^2, beyond the 256 MB Infinity Cache, so the sweeps
@@ -610,6 +677,8 @@ def main():
     ap.add_argument("--probe-n", type=int, default=1024, choices=(1024, 2048))
     ap.add_argument("--fft-probe", action="store_true",
                     help="run only FourierTransform plans of 1024^2, 2048^2, 128^3 (rocprofv3)")
+    ap.add_argument("--serving-probe", type=int, metavar="NREQ",
+                    help="run only the request loop over NREQ requests (for rocprofv3) and exit")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc outputs (CPU)")
     args = ap.parse_args()
@@ -624,6 +693,14 @@ def main():
         return
     if args.fft_probe:
         fft_probe(max(args.probe_steps, 1))
+        return
+    if args.serving_probe:
+        from qusim_amd import _native as Nn
+        Nn.init(0)
+        st = serving_section(args.serving_probe, args.rate, args.hbm_budget_gib,
+                             capacity_count=min(600, args.serving_probe))
+        st.pop("spot_check_fp64", None)
+        print(json.dumps(st), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU and has no multi-GPU path: requests and energies "
@@ -688,6 +765,12 @@ def main():
             out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
             return None
 
+    log("multi-size group section")
+    gs = section("group_2d", group_section)
+    if gs:
+        log("group of 14 2D jobs (256^2..1024^2): %.3e grid-pt-steps/s with graphs, %.3e direct, "
+            "%.3e as per-size batches" % (gs["group_graph"]["value"], gs["group_direct"]["value"],
+                                          gs["per_size_batches"]["value"]))
     log("4096^2 complex128 (beyond the Infinity Cache)")
     g = section("grid_4096", large_grid_section)
     if g:

@@ -18,6 +18,7 @@
 // stored and transformed in complex64; every phase factor and inter-pass
 // twiddle is FP64 and its product with the data is formed in FP64 and rounded
 // once, so the rounding of a factor is never repeated step after step.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -736,21 +737,34 @@ int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps
     }                                                                                 \
     return set_error(QSIM_ERR_UNSUPPORTED, "no fused kernel for this length");
 
-// Launch variants: lengths 1024 and 2048 have the candidates 0..kVariants-1
-// (P2Cfg), every other length only variant 0.
+// Launch variants: lengths 1024 and 2048 have the candidates 0..7, lengths
+// 128, 256 and 512 the two radix plans 0 (16 or 32 elements per thread) and
+// 1 (8 per thread), every other length only variant 0 (P2Cfg).
 constexpr int kVariants = 8;
 
-bool has_variants(size_t n) { return n == 1024 || n == 2048; }
+int num_variants(size_t n)
+{
+    return (n == 1024 || n == 2048) ? kVariants : (n == 128 || n == 256 || n == 512) ? 2 : 1;
+}
+bool has_variants(size_t n) { return num_variants(n) > 1; }
 
 // Variants a new job starts with: the pairs the autotuner picked most often
 // on MI355X for single and batched jobs of these lengths (row variant by the
 // row length ny, column variant by the column length nx).  Fixed, so jobs that
 // are never tuned (request loops, batches) still take a fast pair and two
 // runs of one job always choose the same radix plans.
+// 128- to 512-point lines take the 8-per-thread plan (variant 1): measured
+// on MI355X per step, single job / 16 batched: 128^2 FP32 11.7 -> 9.6 /
+// 13.4 -> 11.2 us, 256^2 FP64 13.7 -> 11.0 / 23.9 -> 20.9 us, 512^2 FP32
+// 25.9 -> 16.1 / 49.0 -> 51.4 us.  (512 FP32 columns in plan 0 batch 13%
+// faster, but their 32 elements per thread would set the register budget of
+// the multi-size group kernels, which mirror these defaults.)
 __host__ __device__ constexpr int default_variant(int prec, size_t n, bool row)
 {
-    return n == 1024 ? (prec == 64 ? 4 : 7)
-                     : n == 2048 ? (prec == 64 ? (row ? 7 : 5) : (row ? 7 : 3)) : 0;
+    return n == 1024   ? (prec == 64 ? 4 : 7)
+           : n == 2048 ? (prec == 64 ? (row ? 7 : 5) : (row ? 7 : 3))
+           : (n == 128 || n == 256 || n == 512) ? 1
+                                                : 0;
 }
 
 // The x-split row sweep depends on the radix plan only (its lines per
@@ -768,6 +782,9 @@ template <typename T, int NN, int MODE, int R>
 int row_vx(int var, const SplitDev *jobs, int njobs, size_t nx, int iev, const void *tw,
            const void *twx, hipStream_t s, bool dry)
 {
+    if constexpr (NN == 128 || NN == 256 || NN == 512)
+        if (var == 1)
+            return launch_row_x<T, NN, MODE, 1, R>(jobs, njobs, nx, iev, tw, twx, s, dry);
     if constexpr (NN == 1024 || NN == 2048) {
         switch (xrow_plan(NN, var)) {
             case 2: return launch_row_x<T, NN, MODE, 2, R>(jobs, njobs, nx, iev, tw, twx, s, dry);
@@ -787,6 +804,8 @@ int row_v(int var, int R, const SplitDev *jobs, int njobs, size_t nrows, int iev
     if (R == 2) return row_vx<T, NN, MODE, 2>(var, jobs, njobs, nrows, iev, tw, twx, s, dry);
     if (R == 4) return row_vx<T, NN, MODE, 4>(var, jobs, njobs, nrows, iev, tw, twx, s, dry);
     if (dry) return QSIM_OK;
+    if constexpr (NN == 128 || NN == 256 || NN == 512)
+        if (var == 1) return launch_row<T, NN, MODE, 1>(jobs, njobs, nrows, iev, tw, s);
     if constexpr (NN == 1024 || NN == 2048) {
         switch (var) {
             case 1: return launch_row<T, NN, MODE, 1>(jobs, njobs, nrows, iev, tw, s);
@@ -806,6 +825,8 @@ template <typename T, int NN, int R>
 int col_vr(int var, const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
            hipStream_t s)
 {
+    if constexpr (NN == 128 || NN == 256 || NN == 512)
+        if (var == 1) return launch_col<T, NN, 1, R>(jobs, njobs, ncols, ik, tw, s);
     if constexpr (NN == 1024 || NN == 2048) {
         switch (var) {
             case 1: return launch_col<T, NN, 1, R>(jobs, njobs, ncols, ik, tw, s);
@@ -1337,10 +1358,10 @@ int qsim_hip_split_set_variants(qsim_split_job *j, int row_variant, int col_vari
     QSIM_CHECK(row_variant >= 0 && row_variant < kVariants && col_variant >= 0 &&
                    col_variant < kVariants,
                "launch variant out of range");
-    const int rv = has_variants(j->d.ny) ? row_variant : 0;
+    const int rv = row_variant < num_variants(j->d.ny) ? row_variant : 0;
     if (j->fused && j->d.dim == 2) QSIM_TRY(xsplit_check(j, j->xsplit, rv));
     j->row_var = rv;
-    j->col_var = has_variants(j->d.nx / j->xsplit) ? col_variant : 0;
+    j->col_var = col_variant < num_variants(j->d.nx / j->xsplit) ? col_variant : 0;
     drop_graphs(j);
     return QSIM_OK;
 }
@@ -1431,7 +1452,7 @@ int autotune_jobs(qsim_split_job *const *jobs, int njobs, int reps, int *row_var
         };
         for (int which = 0; which < 2 && rc == QSIM_OK; ++which) {
             const size_t nlen = which == 0 ? j->d.ny : m;
-            const int nv = has_variants(nlen) ? kVariants : 1;
+            const int nv = num_variants(nlen);
             int chosen = var[which];
             for (int v = 0; v < nv && rc == QSIM_OK; ++v) {
                 const int vr_ = which == 0 ? v : var[0], vc_ = which == 1 ? v : var[1];
@@ -1753,21 +1774,28 @@ int qsim_hip_split_steps_batched_noalloc(qsim_split_job *const *jobs, int njobs,
 }  // extern "C"
 
 // ------------------------------------------------------ multi-size job groups
-// A group advances fused 2D split jobs of different grid sizes together: one
-// launch per sweep for all of them, from a fixed grid of workgroups that
-// loops over a device-side list of (job, tile) items.  Each job keeps its own
-// step program: the row pass of global index P runs job j's FIRST sweep when
-// P == first_pass(j), its LAST when P == first_pass(j) + nph * nsteps(j),
-// a MID sweep in between and nothing otherwise; the column pass P runs while
-// first_pass(j) <= P < first_pass(j) + nph * nsteps(j).  So every job sees
-// exactly the sweep sequence of qsim_hip_split_steps(job, nsteps), in the
-// same radix plans (its default launch variants), and its result is the
-// same bit for bit, whatever else shares the launches.  The pass index base
-// lives in device memory and a final kernel of each advance adds the number
-// of passes: a captured advance stays valid while jobs join and leave.
+// A group advances fused 2D split jobs of different grid sizes together.
+// Jobs are dealt into length classes (the length of the lines a sweep
+// transforms: ny for row sweeps, nx for column sweeps); each class has a
+// kernel of its own (the register allocation of the per-length kernels) that
+// loops, from a fixed grid, over the class's part of a device-side list of
+// (job, tile) items.  Each job keeps its own step program: the row pass of
+// global index P runs job j's FIRST sweep when P == first_pass(j), its LAST
+// when P == first_pass(j) + nph * nsteps(j), a MID sweep in between and
+// nothing otherwise; the column pass P runs while first_pass(j) <= P <
+// first_pass(j) + nph * nsteps(j).  So every job sees exactly the sweep
+// sequence of qsim_hip_split_steps(job, nsteps), in the same radix plans
+// (its default launch variants), and its result is the same bit for bit,
+// whatever else shares the launches.  An advance runs, for each shape class
+// (nx, ny), its chain of passes; the chains are independent, so a captured
+// advance forks them onto parallel branches and joins them before a final
+// kernel adds the number of passes to the device-side pass index.  Work
+// lists, counts and the pass index live in device memory: a captured advance
+// stays valid while jobs of the same shapes join and leave.
 namespace qsim {
 
-constexpr int kGroupThreads = 256;
+constexpr int kGroupLens = 6;      // line lengths 64 .. 2048
+constexpr int kGroupShapes = kGroupLens * kGroupLens;
 
 struct GroupJob {
     SplitDev d;
@@ -1783,123 +1811,85 @@ struct GroupItem {
     unsigned line0; // first row (row pass) or column (column pass) of the tile
 };
 
-// One length's launch shape in the group kernels: the job's default radix
-// plan, kGroupThreads / TPL lines per workgroup, the standalone LDS strides.
-template <typename T, int N, bool LF>
-struct GroupLaunch {
-    static constexpr int V = default_variant(sizeof(T) == 8 ? 64 : 32, N, !LF);
-    using Line = typename P2Cfg<T, N, V>::Line;
-    static_assert(Line::TPL <= kGroupThreads, "one line per group workgroup at most");
-    static constexpr int L = kGroupThreads / Line::TPL;
-    static constexpr int TARGET = LF ? (L >= 16 ? 1 : 16 / L) : 1;
-    static constexpr int LSTR =
-        Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
-    static constexpr size_t LDS_BYTES = (size_t)L * LSTR * sizeof(cx<T>);
+// Per shape class s (nx, ny): its row items at items[off[4s] ..), count
+// off[4s+1]; its column items at items[off[4s+2] ..), count off[4s+3] (a
+// multiple of 8).
+struct GroupCounts {
+    int off[4 * kGroupShapes];
 };
 
-__host__ __device__ constexpr size_t zmax(size_t a, size_t b) { return a > b ? a : b; }
-template <typename T, int N>
-__host__ __device__ constexpr size_t group_lds_n()
-{
-    return zmax(GroupLaunch<T, N, false>::LDS_BYTES, GroupLaunch<T, N, true>::LDS_BYTES);
-}
-// dynamic LDS of the group kernels: the largest over the lengths they take
-template <typename T>
-__host__ __device__ constexpr size_t group_lds()
-{
-    return zmax(zmax(zmax(group_lds_n<T, 64>(), group_lds_n<T, 128>()),
-                     zmax(group_lds_n<T, 256>(), group_lds_n<T, 512>())),
-                zmax(group_lds_n<T, 1024>(), group_lds_n<T, 2048>()));
-}
+// One length's launch shape in the group kernels: that of the standalone
+// sweep kernel in the job's default variant (radix plan, lines per
+// workgroup, LDS strides).
+template <typename T, int N, bool LF>
+struct GroupLaunch : P2Launch<T, N, LF, default_variant(sizeof(T) == 8 ? 64 : 32, N, !LF)> {
+};
 
 __host__ __device__ constexpr bool group_len_ok(size_t n)
 {
     return n >= 64 && n <= 2048 && (n & (n - 1)) == 0;
 }
+__host__ __device__ constexpr int group_len_index(size_t n)
+{
+    return n <= 64 ? 0 : 1 + group_len_index(n / 2);
+}
 
-template <typename T, int N>
-__device__ __forceinline__ void group_row_n(const GroupJob &G, unsigned line0, int mode, int iev,
-                                            cx<T> *lds)
+template <typename T, int N, bool NT>
+__global__ void __launch_bounds__((GroupLaunch<T, N, false>::THREADS))
+    k_row_group(const GroupJob *__restrict__ jobs, const GroupItem *__restrict__ items,
+                const GroupCounts *__restrict__ counts, const long long *__restrict__ basep,
+                int r, int shape)
 {
     using GL = GroupLaunch<T, N, false>;
     using P = typename GL::Line;
-    constexpr bool NT = N <= 1024;
-    if (mode == SW_FIRST)
-        row_lines<T, P, GL::L, GL::LSTR, SW_FIRST, NT>(G.d, line0, G.nx, iev, G.twy, lds);
-    else if (mode == SW_MID)
-        row_lines<T, P, GL::L, GL::LSTR, SW_MID, NT>(G.d, line0, G.nx, iev, G.twy, lds);
-    else
-        row_lines<T, P, GL::L, GL::LSTR, SW_LAST, NT>(G.d, line0, G.nx, iev, G.twy, lds);
-}
-
-template <typename T, int N>
-__device__ __forceinline__ void group_col_n(const GroupJob &G, unsigned line0, int ik, cx<T> *lds)
-{
-    using GL = GroupLaunch<T, N, true>;
-    col_lines<T, typename GL::Line, GL::L, GL::LSTR, (N <= 1024), 1>(G.d, line0, G.ny, ik, G.twx,
-                                                                     lds);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kGroupThreads)
-    k_row_group(const GroupJob *__restrict__ jobs, const GroupItem *__restrict__ items,
-                const int *__restrict__ counts, const long long *__restrict__ basep, int r)
-{
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-    const long long P = *basep + r;
-    const int nitems = counts[0];
+    const long long Pidx = *basep + r;
+    const GroupItem *list = items + counts->off[4 * shape];
+    const int nitems = counts->off[4 * shape + 1];
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-        const GroupItem it = items[w];
+        const GroupItem it = list[w];
         if (it.job < 0) continue;
         const GroupJob &G = jobs[it.job];
-        const long long q = P - G.first_pass, qend = (long long)G.nph * G.nsteps;
+        const long long q = Pidx - G.first_pass, qend = (long long)G.nph * G.nsteps;
         if (q < 0 || q > qend) continue;  // the job is not running at this pass
-        const int mode = q == 0 ? SW_FIRST : (q == qend ? SW_LAST : SW_MID);
-        const int iev = mode == SW_FIRST ? G.ev_start
-                        : mode == SW_LAST ? G.ev_end
-                                          : G.ph_e[(int)((q - 1) % G.nph)];
-        switch (G.ny) {
-            case 64: group_row_n<T, 64>(G, it.line0, mode, iev, lds); break;
-            case 128: group_row_n<T, 128>(G, it.line0, mode, iev, lds); break;
-            case 256: group_row_n<T, 256>(G, it.line0, mode, iev, lds); break;
-            case 512: group_row_n<T, 512>(G, it.line0, mode, iev, lds); break;
-            case 1024: group_row_n<T, 1024>(G, it.line0, mode, iev, lds); break;
-            case 2048: group_row_n<T, 2048>(G, it.line0, mode, iev, lds); break;
-            default: break;
+        if (q == 0) {
+            row_lines<T, P, GL::L, GL::LSTR, SW_FIRST, NT>(G.d, it.line0, G.nx, G.ev_start, G.twy,
+                                                           lds);
+        } else if (q == qend) {
+            row_lines<T, P, GL::L, GL::LSTR, SW_LAST, NT>(G.d, it.line0, G.nx, G.ev_end, G.twy,
+                                                          lds);
+        } else {
+            row_lines<T, P, GL::L, GL::LSTR, SW_MID, NT>(G.d, it.line0, G.nx,
+                                                         G.ph_e[(int)((q - 1) % G.nph)], G.twy,
+                                                         lds);
         }
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kGroupThreads)
+template <typename T, int N, bool NT>
+__global__ void __launch_bounds__((GroupLaunch<T, N, true>::THREADS))
     k_col_group(const GroupJob *__restrict__ jobs, const GroupItem *__restrict__ items,
-                const int *__restrict__ counts, const long long *__restrict__ basep, int r)
+                const GroupCounts *__restrict__ counts, const long long *__restrict__ basep,
+                int r, int shape)
 {
+    using GL = GroupLaunch<T, N, true>;
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-    const long long P = *basep + r;
-    const int nitems = counts[1];  // a multiple of 8 (padded)
-    const GroupItem *cols = items + counts[0];  // after the row items
+    const long long Pidx = *basep + r;
+    const GroupItem *list = items + counts->off[4 * shape + 2];
+    const int nitems = counts->off[4 * shape + 3];  // a multiple of 8 (padded)
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         // the grid is a multiple of 8, so workgroup b (on XCD b % 8) takes
         // items w = b (mod 8): give each XCD a contiguous range of tiles
         const int we = (w % 8) * (nitems / 8) + w / 8;
-        const GroupItem it = cols[we];
+        const GroupItem it = list[we];
         if (it.job < 0) continue;
         const GroupJob &G = jobs[it.job];
-        const long long q = P - G.first_pass;
+        const long long q = Pidx - G.first_pass;
         if (q < 0 || q >= (long long)G.nph * G.nsteps) continue;
-        const int ik = G.ph_k[(int)(q % G.nph)];
-        switch (G.nx) {
-            case 64: group_col_n<T, 64>(G, it.line0, ik, lds); break;
-            case 128: group_col_n<T, 128>(G, it.line0, ik, lds); break;
-            case 256: group_col_n<T, 256>(G, it.line0, ik, lds); break;
-            case 512: group_col_n<T, 512>(G, it.line0, ik, lds); break;
-            case 1024: group_col_n<T, 1024>(G, it.line0, ik, lds); break;
-            case 2048: group_col_n<T, 2048>(G, it.line0, ik, lds); break;
-            default: break;
-        }
+        col_lines<T, typename GL::Line, GL::L, GL::LSTR, NT, 1>(
+            G.d, it.line0, G.ny, G.ph_k[(int)(q % G.nph)], G.twx, lds);
     }
 }
 
@@ -1919,13 +1909,25 @@ struct qsim_split_group {
     int grid = 0;
     // device tables and their pinned staging
     GroupJob *djobs = nullptr, *hjobs = nullptr;
-    GroupItem *dItems = nullptr, *hItems = nullptr;  // row items, then column items
+    GroupItem *dItems = nullptr, *hItems = nullptr;
     size_t item_cap = 0;
-    int *dcounts = nullptr, *hcounts = nullptr;
+    GroupCounts *dcounts = nullptr, *hcounts = nullptr;
     long long *dbase = nullptr;
     bool dirty = true;
     hipEvent_t copied = nullptr;
     bool pending = false;
+    // shape classes with member jobs (a capture holds them: a change of the
+    // set drops the captures)
+    std::vector<int> shapes;
+    // grids of each shape's row and column kernels: its item counts when the
+    // shape set last changed (multiples of 8, at most 8 workgroups per CU);
+    // a shape whose work list grew later loops over it
+    int grid_row[kGroupShapes] = {}, grid_col[kGroupShapes] = {};
+    size_t shape_points[kGroupShapes] = {};
+    // fork / join of the shape chains in direct (uncaptured) advances
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> lane_ev;
+    hipEvent_t fork_ev = nullptr;
     std::map<int, hipGraphExec_t> graphs;  // captured advances, by pass count
 };
 
@@ -1939,29 +1941,137 @@ void group_drop_graphs(qsim_split_group *g)
     g->graphs.clear();
 }
 
+// lines per workgroup and LDS bytes of one length's group kernels
+struct GroupShapeInfo {
+    int lrow, lcol;
+    size_t lds_row, lds_col;
+};
+
 template <typename T>
-int group_launch_passes(qsim_split_group *g, int passes, hipStream_t s)
+GroupShapeInfo group_info_t(size_t nx, size_t ny)
 {
-    const size_t lds = group_lds<T>();
-    QSIM_TRY(lds_limit((const void *)k_row_group<T>, lds));
-    QSIM_TRY(lds_limit((const void *)k_col_group<T>, lds));
+    GroupShapeInfo r{1, 1, 0, 0};
+    switch (ny) {
+#define QSIM_GR(NN)                                            \
+    case NN:                                                   \
+        r.lrow = GroupLaunch<T, NN, false>::L;                 \
+        r.lds_row = GroupLaunch<T, NN, false>::LDS_BYTES;      \
+        break;
+        QSIM_GR(64) QSIM_GR(128) QSIM_GR(256) QSIM_GR(512) QSIM_GR(1024) QSIM_GR(2048)
+#undef QSIM_GR
+        default: break;
+    }
+    switch (nx) {
+#define QSIM_GC(NN)                                            \
+    case NN:                                                   \
+        r.lcol = GroupLaunch<T, NN, true>::L;                  \
+        r.lds_col = GroupLaunch<T, NN, true>::LDS_BYTES;       \
+        break;
+        QSIM_GC(64) QSIM_GC(128) QSIM_GC(256) QSIM_GC(512) QSIM_GC(1024) QSIM_GC(2048)
+#undef QSIM_GC
+        default: break;
+    }
+    return r;
+}
+
+GroupShapeInfo group_info(int prec, size_t nx, size_t ny)
+{
+    return prec == 64 ? group_info_t<double>(nx, ny) : group_info_t<float>(nx, ny);
+}
+
+int group_shape(const qsim_split_job *j)
+{
+    return group_len_index(j->d.nx) * kGroupLens + group_len_index(j->d.ny);
+}
+
+template <typename T, int NX, int NY>
+int group_chain_t(qsim_split_group *g, int shape, int passes, hipStream_t s)
+{
+    using GR = GroupLaunch<T, NY, false>;
+    using GC = GroupLaunch<T, NX, true>;
+    // non-temporal line loads by the rule of the standalone sweeps (lines of
+    // up to 1024 points, 8 MiB or more per launch), on the shape's lines when
+    // the chain was set up
+    const size_t bytes = g->shape_points[shape] * sizeof(cx<T>);
+    auto kr = k_row_group<T, NY, false>;
+    auto kc = k_col_group<T, NX, false>;
+    if constexpr (NY <= 1024)
+        if (line_nt(NY, bytes)) kr = k_row_group<T, NY, true>;
+    if constexpr (NX <= 1024)
+        if (line_nt(NX, bytes)) kc = k_col_group<T, NX, true>;
+    QSIM_TRY(lds_limit((const void *)kr, GR::LDS_BYTES));
+    QSIM_TRY(lds_limit((const void *)kc, GC::LDS_BYTES));
     for (int r = 0; r < passes; ++r) {
-        hipLaunchKernelGGL(k_row_group<T>, dim3(g->grid), dim3(kGroupThreads), lds, s, g->djobs,
-                           g->dItems, g->dcounts, g->dbase, r);
+        hipLaunchKernelGGL(kr, dim3(g->grid_row[shape]), dim3(GR::THREADS), GR::LDS_BYTES, s,
+                           g->djobs, g->dItems, g->dcounts, g->dbase, r, shape);
         QSIM_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_col_group<T>, dim3(g->grid), dim3(kGroupThreads), lds, s, g->djobs,
-                           g->dItems, g->dcounts, g->dbase, r);
+        hipLaunchKernelGGL(kc, dim3(g->grid_col[shape]), dim3(GC::THREADS), GC::LDS_BYTES, s,
+                           g->djobs, g->dItems, g->dcounts, g->dbase, r, shape);
         QSIM_HIP(hipGetLastError());
+    }
+    return QSIM_OK;
+}
+
+template <typename T, int NX>
+int group_chain_nx(qsim_split_group *g, int shape, int passes, hipStream_t s)
+{
+    switch (shape % kGroupLens) {
+        case 0: return group_chain_t<T, NX, 64>(g, shape, passes, s);
+        case 1: return group_chain_t<T, NX, 128>(g, shape, passes, s);
+        case 2: return group_chain_t<T, NX, 256>(g, shape, passes, s);
+        case 3: return group_chain_t<T, NX, 512>(g, shape, passes, s);
+        case 4: return group_chain_t<T, NX, 1024>(g, shape, passes, s);
+        default: return group_chain_t<T, NX, 2048>(g, shape, passes, s);
+    }
+}
+
+template <typename T>
+int group_chain_prec(qsim_split_group *g, int shape, int passes, hipStream_t s)
+{
+    switch (shape / kGroupLens) {
+        case 0: return group_chain_nx<T, 64>(g, shape, passes, s);
+        case 1: return group_chain_nx<T, 128>(g, shape, passes, s);
+        case 2: return group_chain_nx<T, 256>(g, shape, passes, s);
+        case 3: return group_chain_nx<T, 512>(g, shape, passes, s);
+        case 4: return group_chain_nx<T, 1024>(g, shape, passes, s);
+        default: return group_chain_nx<T, 2048>(g, shape, passes, s);
+    }
+}
+
+int group_chain(qsim_split_group *g, int shape, int passes, hipStream_t s)
+{
+    return g->prec == 64 ? group_chain_prec<double>(g, shape, passes, s)
+                         : group_chain_prec<float>(g, shape, passes, s);
+}
+
+// One advance on stream s: the shape chains forked onto the group's lanes
+// (or, under capture, onto parallel branches of the graph) and joined before
+// the pass index moves on.
+int group_enqueue(qsim_split_group *g, int passes, hipStream_t s)
+{
+    const int ns = (int)g->shapes.size();
+    while ((int)g->lanes.size() < ns) {
+        hipStream_t ls;
+        hipEvent_t le;
+        QSIM_HIP(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
+        g->lanes.push_back(ls);
+        QSIM_HIP(hipEventCreateWithFlags(&le, hipEventDisableTiming));
+        g->lane_ev.push_back(le);
+    }
+    if (ns == 1) {
+        QSIM_TRY(group_chain(g, g->shapes[0], passes, s));
+    } else if (ns > 1) {
+        QSIM_HIP(hipEventRecord(g->fork_ev, s));
+        for (int c = 0; c < ns; ++c) {
+            QSIM_HIP(hipStreamWaitEvent(g->lanes[c], g->fork_ev, 0));
+            QSIM_TRY(group_chain(g, g->shapes[c], passes, g->lanes[c]));
+            QSIM_HIP(hipEventRecord(g->lane_ev[c], g->lanes[c]));
+            QSIM_HIP(hipStreamWaitEvent(s, g->lane_ev[c], 0));
+        }
     }
     hipLaunchKernelGGL(k_group_tick, dim3(1), dim3(64), 0, s, g->dbase, passes);
     QSIM_HIP(hipGetLastError());
     return QSIM_OK;
-}
-
-int group_enqueue(qsim_split_group *g, int passes, hipStream_t s)
-{
-    return g->prec == 64 ? group_launch_passes<double>(g, passes, s)
-                         : group_launch_passes<float>(g, passes, s);
 }
 
 // Rebuilds the work lists from the member jobs and uploads them (stream
@@ -1969,33 +2079,43 @@ int group_enqueue(qsim_split_group *g, int passes, hipStream_t s)
 int group_upload(qsim_split_group *g, hipStream_t s)
 {
     const int nj = (int)g->jobs.size();
-    size_t nrow = 0, ncol = 0;
-    std::vector<int> lrow(nj), lcol(nj);
-    for (int i = 0; i < nj; ++i) {
-        const qsim_split_job *j = g->jobs[i];
-        const int prec = g->prec;
-        auto lines = [prec](size_t n, bool lf) -> int {
-            switch (n) {
-#define QSIM_GL(NN)                                                                            \
-    case NN:                                                                                   \
-        return prec == 64 ? (lf ? GroupLaunch<double, NN, true>::L : GroupLaunch<double, NN, false>::L) \
-                          : (lf ? GroupLaunch<float, NN, true>::L : GroupLaunch<float, NN, false>::L);
-                QSIM_GL(64) QSIM_GL(128) QSIM_GL(256) QSIM_GL(512) QSIM_GL(1024) QSIM_GL(2048)
-#undef QSIM_GL
-                default: return 1;
-            }
-        };
-        lrow[i] = lines(j->d.ny, false);
-        lcol[i] = lines(j->d.nx, true);
-        nrow += (j->d.nx + lrow[i] - 1) / lrow[i];
-        ncol += (j->d.ny + lcol[i] - 1) / lcol[i];
+    std::vector<std::vector<int>> by(kGroupShapes);
+    for (int i = 0; i < nj; ++i) by[group_shape(g->jobs[i])].push_back(i);
+    std::vector<int> shapes;
+    size_t need = 0;
+    size_t nrow_of[kGroupShapes] = {}, ncol_of[kGroupShapes] = {};
+    for (int sh = 0; sh < kGroupShapes; ++sh) {
+        if (by[sh].empty()) continue;
+        shapes.push_back(sh);
+        size_t nrow = 0, ncol = 0;
+        for (int i : by[sh]) {
+            const qsim_split_job *j = g->jobs[i];
+            const GroupShapeInfo inf = group_info(g->prec, j->d.nx, j->d.ny);
+            nrow += (j->d.nx + inf.lrow - 1) / inf.lrow;
+            ncol += (j->d.ny + inf.lcol - 1) / inf.lcol;
+        }
+        nrow_of[sh] = nrow;
+        ncol_of[sh] = (ncol + 7) / 8 * 8;
+        need += nrow + ncol_of[sh];
     }
-    ncol = (ncol + 7) / 8 * 8;  // column items padded to a multiple of 8
-    const size_t need = nrow + ncol;
+    if (shapes != g->shapes) {
+        group_drop_graphs(g);
+        g->shapes = shapes;
+        auto fit = [g](size_t n) {
+            const size_t r = std::min<size_t>((n + 7) / 8 * 8, (size_t)g->grid);
+            return (int)std::max<size_t>(r, 8);
+        };
+        for (int sh : shapes) {
+            g->grid_row[sh] = fit(nrow_of[sh]);
+            g->grid_col[sh] = fit(ncol_of[sh]);
+            g->shape_points[sh] = 0;
+            for (int i : by[sh]) g->shape_points[sh] += g->jobs[i]->n;
+        }
+    }
     if (g->pending) QSIM_HIP(hipEventSynchronize(g->copied));
     g->pending = false;
     if (need > g->item_cap) {
-        // new device tables: captured advances hold the old pointers
+        // new device tables: captured advances hold the old pointer
         group_drop_graphs(g);
         if (g->dItems) QSIM_HIP(hipFree(g->dItems));
         if (g->hItems) QSIM_HIP(hipHostFree(g->hItems));
@@ -2007,7 +2127,6 @@ int group_upload(qsim_split_group *g, hipStream_t s)
         QSIM_HIP(hipHostMalloc(&g->hItems, cap * sizeof(GroupItem), hipHostMallocDefault));
         g->item_cap = cap;
     }
-    size_t w = 0;
     for (int i = 0; i < nj; ++i) {
         qsim_split_job *j = g->jobs[i];
         GroupJob &G = g->hjobs[i];
@@ -2025,21 +2144,34 @@ int group_upload(qsim_split_group *g, hipStream_t s)
         }
         G.first_pass = g->first[i];
         G.nsteps = g->nsteps[i];
-        for (size_t r0 = 0; r0 < j->d.nx; r0 += lrow[i]) g->hItems[w++] = GroupItem{i, (unsigned)r0};
     }
-    for (int i = 0; i < nj; ++i)
-        for (size_t c0 = 0; c0 < g->jobs[i]->d.ny; c0 += lcol[i])
-            g->hItems[w++] = GroupItem{i, (unsigned)c0};
-    while (w < need) g->hItems[w++] = GroupItem{-1, 0};
-    g->hcounts[0] = (int)nrow;
-    g->hcounts[1] = (int)ncol;
+    std::memset(g->hcounts, 0, sizeof(GroupCounts));
+    size_t w = 0;
+    for (int sh : shapes) {
+        g->hcounts->off[4 * sh] = (int)w;
+        for (int i : by[sh]) {
+            const qsim_split_job *j = g->jobs[i];
+            const int L = group_info(g->prec, j->d.nx, j->d.ny).lrow;
+            for (size_t r0 = 0; r0 < j->d.nx; r0 += L) g->hItems[w++] = GroupItem{i, (unsigned)r0};
+        }
+        g->hcounts->off[4 * sh + 1] = (int)(w - g->hcounts->off[4 * sh]);
+        g->hcounts->off[4 * sh + 2] = (int)w;
+        for (int i : by[sh]) {
+            const qsim_split_job *j = g->jobs[i];
+            const int L = group_info(g->prec, j->d.nx, j->d.ny).lcol;
+            for (size_t c0 = 0; c0 < j->d.ny; c0 += L) g->hItems[w++] = GroupItem{i, (unsigned)c0};
+        }
+        while ((w - g->hcounts->off[4 * sh + 2]) % 8) g->hItems[w++] = GroupItem{-1, 0};
+        g->hcounts->off[4 * sh + 3] = (int)(w - g->hcounts->off[4 * sh + 2]);
+    }
     if (nj)
         QSIM_HIP(hipMemcpyAsync(g->djobs, g->hjobs, sizeof(GroupJob) * nj, hipMemcpyHostToDevice,
                                 s));
-    if (need)
-        QSIM_HIP(hipMemcpyAsync(g->dItems, g->hItems, sizeof(GroupItem) * need,
+    if (w)
+        QSIM_HIP(hipMemcpyAsync(g->dItems, g->hItems, sizeof(GroupItem) * w,
                                 hipMemcpyHostToDevice, s));
-    QSIM_HIP(hipMemcpyAsync(g->dcounts, g->hcounts, sizeof(int) * 2, hipMemcpyHostToDevice, s));
+    QSIM_HIP(hipMemcpyAsync(g->dcounts, g->hcounts, sizeof(GroupCounts), hipMemcpyHostToDevice,
+                            s));
     QSIM_HIP(hipEventRecord(g->copied, s));
     g->pending = true;
     g->dirty = false;
@@ -2062,6 +2194,10 @@ int qsim_hip_split_group_destroy(qsim_split_group *g)
     if (!g) return QSIM_OK;
     if (g->pending) (void)hipEventSynchronize(g->copied);
     group_drop_graphs(g);
+    (void)hipDeviceSynchronize();
+    for (hipStream_t ls : g->lanes) (void)hipStreamDestroy(ls);
+    for (hipEvent_t le : g->lane_ev) (void)hipEventDestroy(le);
+    if (g->fork_ev) (void)hipEventDestroy(g->fork_ev);
     if (g->copied) (void)hipEventDestroy(g->copied);
     if (g->djobs) (void)hipFree(g->djobs);
     if (g->hjobs) (void)hipHostFree(g->hjobs);
@@ -2085,15 +2221,17 @@ int qsim_hip_split_group_create(qsim_split_group **out, int capacity, int precis
     g->method = method;
     g->cap = capacity;
     const int cu = device_cu_count();
-    g->grid = ((cu > 0 ? 2 * cu : 512) + 7) / 8 * 8;
+    // the largest grid of a shape chain's kernels: eight workgroups per CU
+    g->grid = ((cu > 0 ? 8 * cu : 2048) + 7) / 8 * 8;
     int rc = QSIM_OK;
     const long long zero = 0;
     if (hipMalloc(&g->djobs, sizeof(GroupJob) * capacity) != hipSuccess ||
         hipHostMalloc(&g->hjobs, sizeof(GroupJob) * capacity, hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&g->dcounts, sizeof(int) * 2) != hipSuccess ||
-        hipHostMalloc(&g->hcounts, sizeof(int) * 2, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&g->dcounts, sizeof(GroupCounts)) != hipSuccess ||
+        hipHostMalloc(&g->hcounts, sizeof(GroupCounts), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&g->dbase, sizeof(long long)) != hipSuccess ||
         hipEventCreateWithFlags(&g->copied, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->fork_ev, hipEventDisableTiming) != hipSuccess ||
         hipMemcpy(g->dbase, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipGetLastError();
         rc = set_error(QSIM_ERR_NOMEM, "group: allocation failed");

@@ -8,17 +8,21 @@ the device at once instead:
 * Requests arrive over time.  A request is admitted (its grids built on the
   GPU and its job created) as soon as its HBM footprint fits the budget,
   first come first served.
-* Admitted split jobs join a lane of their shape: (kind, n, precision).
-  A lane advances all its jobs in one launch per sweep
-  (qsim_hip_split_batch_steps) on its own stream, chunk by chunk.  New jobs
-  join the lane between chunks, and finished jobs leave it, while the other
-  lanes keep running.
+* Admitted 2D evolutions of every grid size share one multi-size group
+  (qsim_hip_split_group_*) on a stream of its own: each advance is one
+  launch per sweep for all of them, replayed from a captured HIP graph, and
+  each job runs its whole step count as one sweep program spread over as
+  many advances as it needs.  New jobs join and finished jobs leave between
+  advances.
+* 1D evolutions join a lane of their length: a lane advances all its jobs
+  in one launch per chunk (qsim_hip_split_batch_steps) on its own stream;
+  jobs join and leave between chunks.
 * Born solves run on worker threads, each with a stream of its own; each
   solve advances a fixed number of iterations per turn.
-* A job advances in chunks of a fixed length set by its kind and size
-  (`quantum`), never by what else is on the device.  The batched kernels
-  compute every job independently.  So a result is bitwise the same alone
-  and in any mix.
+* A 1D job advances in chunks of a fixed length set by its kind and size
+  (`quantum`), a 2D job in one program of its own step count, never by what
+  else is on the device, and the kernels compute every job independently.
+  So a result is bitwise the same alone and in any mix.
 
 Footprints (bytes per grid point), which bound what a job allocates:
   evolution FP32-mixed: psi 16 + potential staging 16 + phase tables 32 + FP32 state 8 = 72
@@ -312,6 +316,9 @@ class _Lane:
         self.jobs = []
         self.inflight = None
 
+    def add(self, job):
+        self.jobs.append(job)
+
     def launch(self):
         groups = collections.OrderedDict()
         for j in self.jobs:
@@ -350,6 +357,65 @@ class _Lane:
         if self.inflight is not None:
             self.event.synchronize()
         N.call("qsim_hip_split_batch_destroy", self.batch)
+        N.call("qsim_hip_stream_destroy", self.stream)
+
+
+def group_ok(req, precision):
+    """2D evolutions whose grid a multi-size group takes (axes 64..2048)."""
+    return req.kind == "evo2d" and 64 <= req.n <= 2048 and fused(req, precision)
+
+
+class _GroupLane:
+    """Every 2D evolution, whatever its grid size, in one multi-size group on
+    one stream.  A job joins with its whole step count and runs it as one
+    sweep program spread over the group's advances (PASSES passes each,
+    replayed from a captured HIP graph), so its result is bitwise that of
+    the job run alone in one call, whatever shares the launches; jobs join
+    and leave between advances."""
+
+    PASSES = 20
+    CAPACITY = 4096
+
+    def __init__(self, precision, method=2):
+        p = ctypes.c_void_p()
+        N.call("qsim_hip_stream_create", ctypes.byref(p))
+        self.stream = p.value
+        g = ctypes.c_void_p()
+        N.call("qsim_hip_split_group_create", ctypes.byref(g), self.CAPACITY, precision, method)
+        self.group = g.value
+        self.event = N.Event()
+        self.jobs = []
+        self.inflight = None
+
+    def add(self, job):
+        N.call("qsim_hip_split_group_add", self.group, job.job, job.req.work)
+        self.jobs.append(job)
+
+    def launch(self):
+        N.call("qsim_hip_split_group_advance", self.group, self.PASSES, 1, self.stream)
+        self.event.record(self.stream)
+        self.inflight = True
+
+    def poll(self):
+        """jobs finished by the completed advance (removed from the group)."""
+        if self.inflight is None or not self.event.query():
+            return None
+        self.inflight = None
+        done, keep = [], []
+        fin = ctypes.c_int()
+        for j in self.jobs:
+            N.call("qsim_hip_split_group_status", self.group, j.job, ctypes.byref(fin), None)
+            (done if fin.value else keep).append(j)
+        for j in done:
+            N.call("qsim_hip_split_group_remove", self.group, j.job)
+            j.done = j.req.work
+        self.jobs = keep
+        return done
+
+    def close(self):
+        if self.inflight is not None:
+            self.event.synchronize()
+        N.call("qsim_hip_split_group_destroy", self.group)
         N.call("qsim_hip_stream_destroy", self.stream)
 
 
@@ -512,10 +578,15 @@ class Server:
                         self._finish(r.rid, t0)
                         self._release(fp)
                         continue
-                    key = (r.kind, r.n)
-                    if key not in lanes:
-                        lanes[key] = _Lane(self.quantum(r))
-                    lanes[key].jobs.append(job)
+                    if group_ok(r, self.precision):
+                        key = "evo2d-group"
+                        if key not in lanes:
+                            lanes[key] = _GroupLane(self.precision)
+                    else:
+                        key = (r.kind, r.n)
+                        if key not in lanes:
+                            lanes[key] = _Lane(self.quantum(r))
+                    lanes[key].add(job)
                 busy = False
                 for lane in lanes.values():
                     done = lane.poll()

@@ -97,7 +97,6 @@ def test_group_refuses_what_it_cannot_run():
         f, _, _ = _job((128, 128), 1, "mixed32", 2)
         with pytest.raises(N.QsimError, match="precision and method"):
             N.call("qsim_hip_split_group_add", g, f._job, 5)
-        N.call("qsim_hip_split_set_variants", e._job, 0, 0)   # 256: default anyway
         big, _, _ = _job((1024, 1024), 2, "fp64", 2)
         N.call("qsim_hip_split_set_variants", big._job, 6, 6)
         with pytest.raises(N.QsimError, match="default launch schedule"):

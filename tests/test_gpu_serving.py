@@ -89,3 +89,45 @@ def test_fp32_results_against_fp64():
         err = compare(r.kind, s.results[r.rid], ref)
         print(r, "FP32 vs FP64 %.2e" % err)
         assert err < (1e-4 if r.kind == "born2d" else 1e-5)
+
+
+def test_group_lane_result_is_one_call_of_the_whole_run():
+    """A 2D evolution in the request loop runs its whole step count as one
+    sweep program of the multi-size group: its psi equals one
+    qsim_hip_split_steps call of all its steps on the job alone."""
+    from qusim_amd import _native as N
+    from qusim_amd.serving import EvoJob, Server, _Staging, make_requests
+    reqs = [r for r in make_requests(120, rate=1e6, seed=21) if r.kind == "evo2d"][:4]
+    for r in reqs:
+        r.work = max(10, r.work // 5)
+    s = Server(precision=32, realtime=False, keep=[r.rid for r in reqs])
+    st = s.run(reqs)
+    assert st["completed"] == len(reqs)
+    staging = _Staging()
+    for r in reqs:
+        j = EvoJob(r, 32, staging)
+        N.call("qsim_hip_split_steps", j.job, r.work, None)
+        N.synchronize()
+        assert np.array_equal(j.psi.download(), s.results[r.rid]["psi"]), (r.n, r.work)
+        j.close()
+    staging.close()
+
+
+def test_full_size_mix_spot_checks():
+    """300 requests of the benchmark mix at full size (steps and iterations
+    as drawn), all at t = 0: every request completes and every 50th,
+    re-solved alone in FP64, agrees within the stated bounds (evolutions
+    1e-5 psi relative L2, Born 1e-4 of the forward cross section)."""
+    from qusim_amd.serving import Server, make_requests, spot_check
+    reqs = make_requests(300, rate=1e6, seed=22)
+    keep = [r.rid for r in reqs[::50]]
+    s = Server(precision=32, realtime=False, keep=keep)
+    st = s.run(reqs)
+    print({k: st[k] for k in ("completed", "solves_per_s", "latency_p50_s", "latency_p99_s")})
+    assert st["completed"] == len(reqs) and st["errors"] == 0
+    sc = spot_check(reqs, s.results, every=50)
+    print(sc)
+    assert sum(v["checked"] for v in sc.values()) == len(keep)
+    for kind, v in sc.items():
+        if v["checked"]:
+            assert v["max"] < (1e-4 if kind == "born2d" else 1e-5), (kind, v)
