@@ -302,3 +302,49 @@ def test_complex128_energies_with_streaming_loads_equal_single_runs(oracle):
             ref, _, _, _ = oracle.born(V, lo, hi, E, (0, 0, 1), order=5, preconditional=1,
                                        preconditioner=1, slow=0.5, threads=8)
             assert rel_l2(got, ref) < 1e-12, E
+
+
+def test_c4_full_job_stop_rule():
+    """The stopping rule at C4's full size (128^3, 32 energies, complex64
+    iterates, FP64 relative-update residual, checked every 4 iterations).
+    With the configured tolerance 1e-8 no energy stops within 200
+    iterations: the Vellekoop series contracts by only 0.987-0.996 per
+    iteration here (epsilon is set by the absorbing layer's corners,
+    3 x 0.8 E), so even in FP64 the relative update is still ~5e-3 after 200
+    iterations.  With 8e-3 the energies stop as they reach it: the active
+    table shrinks inside one call, and every stopped energy's residual is
+    below the tolerance after 4m - 1 updates (every 4th pass reads the
+    residual of the update before it, which the energy keeps); an energy
+    that stopped early equals, bit for bit, a single-energy job run for
+    exactly that many iterations."""
+    from qusim_amd.api import Options, QuPerturbation3D, SolverMethod
+    from qusim_amd.workloads import c4_energies, c4_potential
+    n, L = 128, 32.0
+    V, Es = c4_potential(n=n, L=L), c4_energies()
+
+    def job(energies):
+        o = Options().Hip().Order(1).Preconditional(True).VellekoopPreconditioner().Slow(0.5)
+        o.Precision("mixed32")
+        return QuPerturbation3D().init(V, -L, L, n, -L, L, n, -L, L, n, energies, 0.0, 0, 0, 1,
+                                       SolverMethod.BornSerise, 1.0, 1.0, o)
+
+    s = job(Es)
+    s.Compute(max_iter=200, tol=1e-8)
+    assert list(s.iters) == [200] * 32 and min(s.residual) > 1e-8
+    s.close()
+    tol = 8e-3
+    s = job(Es)
+    s.Compute(max_iter=200, tol=tol)
+    its, res = list(s.iters), list(s.residual)
+    print("C4 stop at 8e-3: iterations", its)
+    early = [e for e in range(32) if its[e] < 200]
+    assert len(early) >= 4
+    for e in range(32):
+        if its[e] < 200:
+            assert res[e] < tol and its[e] % 4 == 3, (e, its[e], res[e])
+    e = early[0]
+    one = job(Es[e])
+    one.Compute(max_iter=its[e], tol=0.0)
+    assert np.array_equal(one.GetPsi(), s.GetPsi(e))
+    one.close()
+    s.close()
