@@ -145,13 +145,17 @@ def host_threads(cap=16):
     return max(1, min(cap, n))
 
 
-def traffic_from_profiles(name):
-    """HBM bytes per launch of `name` from the committed PMC summary, or None."""
+def traffic_from_profiles(name, schedule):
+    """HBM bytes per launch of `name` from the committed PMC summary, when it
+    was measured with this launch schedule (row, column, x-split); else None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        ent = json.load(open(path))["kernels"].get(name)
+        t = json.load(open(path))
+        if list(t.get("schedule", [])) != list(schedule):
+            return None
+        ent = t["kernels"].get(name)
         return None if ent is None else ent["bytes_per_launch"]
     except Exception:
         return None
@@ -159,13 +163,20 @@ def traffic_from_profiles(name):
 
 # kernels the PMC summary reports: (key, substring of the rocprof kernel name)
 PROBE_COPY_N = 1 << 20          # complex128 elements per calibration copy (16 MiB each way)
-PROBE_VARIANTS = (6, 4)         # row / column launch variants of the traffic probe
+# launch schedule of the traffic probe (row variant, column variant, x-split
+# factor): the one the autotuner picks for the C2 job on MI355X
+PROBE_SCHEDULE = (6, 0, 2)
 
 
-def probe_kernels(rv, cv):
+def probe_kernels(rv, cv, xs):
+    """(key, substring of the rocprof kernel name) of the probe's sweeps"""
+    if xs == 1:
+        return (("copy", "qsim_hip_copy"),
+                ("row_sweep", "k_row_sweep<double, 1024, 1, %d," % rv),
+                ("col_sweep", "k_col_sweep<double, 1024, %d," % cv))
     return (("copy", "qsim_hip_copy"),
-            ("row_sweep", "k_row_sweep<double, 1024, 1, %d," % rv),
-            ("col_sweep", "k_col_sweep<double, 1024, %d," % cv))
+            ("row_sweep", "k_row_sweep_x<double, 1024, 1, %d," % rv),
+            ("col_sweep", "k_col_sweep<double, %d, %d," % (1024 // xs, cv)))
 
 
 def batch_probe(steps, njobs, variants, n=1024):
@@ -225,12 +236,14 @@ def traffic_probe(steps):
         Nn.call("qsim_hip_copy", b.ptr, a.ptr, PROBE_COPY_N, None)
     Nn.synchronize()
     job, keep = make_job(Nn, c2_problem(), 64)
-    Nn.call("qsim_hip_split_set_variants", job, PROBE_VARIANTS[0], PROBE_VARIANTS[1])
+    rv, cv, xs = PROBE_SCHEDULE
+    Nn.call("qsim_hip_split_set_xsplit", job, xs)
+    Nn.call("qsim_hip_split_set_variants", job, rv, cv)
     Nn.call("qsim_hip_split_steps", job, steps, None)
     Nn.synchronize()
     Nn.call("qsim_hip_split_destroy", job)
-    log("traffic probe: 3 copies of %d complex128, %d C2 steps, variants %s"
-        % (PROBE_COPY_N, steps, PROBE_VARIANTS))
+    log("traffic probe: 3 copies of %d complex128, %d C2 steps, schedule %s"
+        % (PROBE_COPY_N, steps, PROBE_SCHEDULE))
 
 
 def _pmc_rows(d):
@@ -263,13 +276,13 @@ def pmc_summary(fetch_dir, write_dir):
     kernel of known size."""
     fr, wr = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
     copy_kib = PROBE_COPY_N * 16 / 1024.0
-    kern = probe_kernels(*PROBE_VARIANTS)
+    kern = probe_kernels(*PROBE_SCHEDULE)
     cf, nf = _pmc_avg(fr, "FETCH_SIZE", kern[0][1])
     cw, nw = _pmc_avg(wr, "WRITE_SIZE", kern[0][1])
     out = {"calibration": {"copy_bytes_each_way": copy_kib * 1024, "copy_fetch_kib": cf,
                            "copy_write_kib": cw, "fetch_factor": copy_kib / cf,
                            "write_factor": copy_kib / cw, "dispatches": [nf, nw]},
-           "variants": list(PROBE_VARIANTS), "kernels": {}}
+           "schedule": list(PROBE_SCHEDULE), "kernels": {}}
     alg = {"row_sweep": ROW_B_PER_PT[64] * (1 << 20), "col_sweep": COL_B_PER_PT[64] * (1 << 20)}
     for key, sub in kern[1:]:
         f, n1 = _pmc_avg(fr, "FETCH_SIZE", sub)
@@ -748,7 +761,8 @@ def main():
                    "device": info["name"], "launch_variants": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles(dom),
+                     "traffic": traffic_from_profiles(
+                         dom, [tuned["row_variant"], tuned["col_variant"], tuned["xsplit"]]),
                      "step_algorithmic_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
                      "kernels": per},
     }

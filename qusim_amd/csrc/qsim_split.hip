@@ -233,7 +233,8 @@ __device__ __forceinline__ void xrow_dft(cx<T> (&v)[E], int k)
 
 // Row sweep of the x-split schedule: one group of R rows {i2 + M k} per
 // workgroup (R * P::TPL threads, lane R t + k on row k of the group).
-template <typename T, typename P, int R, int LSTR, int MODE, bool NT>
+// EVP: load the MID sweep's phase factors before the inverse transform.
+template <typename T, typename P, int R, int LSTR, int MODE, bool NT, bool EVP>
 __device__ __forceinline__ void row_group_x(const SplitDev &J, size_t i2, size_t nx, int iev,
                                             const c128 *__restrict__ tw,
                                             const c128 *__restrict__ twx, cx<T> *lds)
@@ -268,7 +269,7 @@ __device__ __forceinline__ void row_group_x(const SplitDev &J, size_t i2, size_t
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
                 v[u * RZ + r] = ld_line<NT>(work + ybase + P::template in_pos<RZ>(t, u, r));
-        constexpr bool PREFETCH = P::E <= 16;
+        constexpr bool PREFETCH = EVP;
         c128 evp[PREFETCH ? P::E : 1];
         if constexpr (PREFETCH) {
 #pragma unroll
@@ -313,18 +314,173 @@ __device__ __forceinline__ void row_group_x(const SplitDev &J, size_t i2, size_t
     }
 }
 
+// The R-point transforms of xrow_dft with the R rows of a group in one
+// thread (v[k] is what lane k holds there): the same butterflies, partner
+// first, so the results are those of the lane exchanges bit for bit.
+template <typename T>
+__device__ __forceinline__ cx<T> lbfly(cx<T> partner, cx<T> mine, bool upper)
+{
+    const T sg = upper ? T(-1) : T(1);
+    return mk(partner.x + sg * mine.x, partner.y + sg * mine.y);
+}
+
+template <int R, bool FWD, typename T, int E>
+__device__ __forceinline__ void xrow_dft_local(cx<T> (&v)[R][E])
+{
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if constexpr (R == 2) {
+            const cx<T> a0 = v[0][e], a1 = v[1][e];
+            v[0][e] = lbfly(a1, a0, false);
+            v[1][e] = lbfly(a0, a1, true);
+        } else {
+            static_assert(R == 4, "row groups of 2 or 4");
+            constexpr int M1 = FWD ? 1 : 2, M2 = FWD ? 2 : 1;
+            cx<T> a[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) a[k] = lbfly(v[k ^ M1][e], v[k][e], (k & M1) != 0);
+            a[3] = FWD ? mk(a[3].y, -a[3].x) : mk(-a[3].y, a[3].x);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k][e] = lbfly(a[k ^ M2], a[k], (k & M2) != 0);
+        }
+    }
+}
+
+// Row sweep of the x-split schedule with the R rows of a group in each
+// thread (P::TPL threads, thread t holds the line elements of t in every
+// row): the cross-row transforms need no lane exchange, and the R line
+// transforms run one after the other through one LDS line.  Rows of 4096
+// points take this layout: one lane per row needs R * TPL threads with a
+// half or a quarter of the register budget each, and those groups spilled
+// to scratch.  Results equal those of row_group_x.
+template <typename T, typename P, int R, int MODE, bool NT, bool EVP>
+__device__ __forceinline__ void row_group_xl(const SplitDev &J, size_t i2, size_t nx, int iev,
+                                             const c128 *__restrict__ tw,
+                                             const c128 *__restrict__ twx, cx<T> *lds)
+{
+    constexpr int N = P::TPL * P::E;
+    const int t = threadIdx.x;
+    const size_t M = nx / R;
+    const c128 *__restrict__ ev = J.ev[iev];
+    cx<T> *work = reinterpret_cast<cx<T> *>(J.work);
+    c128 *psi = J.psi;
+    constexpr int RA = P::RA_, RZ = P::RLAST;
+    // v[k]: the row of lane k in row_group_x (natural row M bitrev(k) + i2,
+    // working-array row M k + i2)
+    cx<T> v[R][P::E];
+    c128 wk[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) wk[k] = ld_g(twx + i2 * k);
+    auto xrow = [&](int k) {
+        const int kr = R == 4 ? ((k & 1) << 1) | (k >> 1) : k;
+        return M * kr + i2;
+    };
+    if constexpr (MODE == SW_FIRST) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const size_t base = xrow(k) * N;
+#pragma unroll
+            for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                for (int r = 0; r < RA; ++r) {
+                    const int pos = P::template in_pos<RA>(t, u, r);
+                    v[k][u * RA + r] =
+                        apply(conv<T>(ld_g(psi + base + pos)), ld_g(ev + base + pos));
+                }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const size_t ybase = (M * k + i2) * N;
+#pragma unroll
+            for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+                for (int r = 0; r < RZ; ++r)
+                    v[k][u * RZ + r] =
+                        ld_line<NT>(work + ybase + P::template in_pos<RZ>(t, u, r));
+        }
+        c128 evp[EVP ? R : 1][EVP ? P::E : 1];
+        if constexpr (EVP) {
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+#pragma unroll
+                for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                    for (int r = 0; r < RA; ++r)
+                        evp[k][u * RA + r] =
+                            ld_g(ev + xrow(k) * N + P::template in_pos<RA>(t, u, r));
+        }
+        // A': w_nx^-(i2 k1), then the inverse R-point transform across rows
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+#pragma unroll
+            for (int e = 0; e < P::E; ++e) v[k][e] = apply(v[k][e], mk(wk[k].x, -wk[k].y));
+        xrow_dft_local<R, false>(v);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            P::inverse(v[k], t, lds, tw);
+            const size_t xbase = xrow(k) * N;
+#pragma unroll
+            for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                for (int r = 0; r < RA; ++r) {
+                    const int pos = P::template in_pos<RA>(t, u, r);
+                    if constexpr (EVP) v[k][u * RA + r] = apply(v[k][u * RA + r], evp[k][u * RA + r]);
+                    else v[k][u * RA + r] = apply(v[k][u * RA + r], ld_g(ev + xbase + pos));
+                }
+        }
+    }
+    if constexpr (MODE == SW_LAST) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const size_t base = xrow(k) * N;
+#pragma unroll
+            for (int u = 0; u < P::E / RA; ++u)
+#pragma unroll
+                for (int r = 0; r < RA; ++r)
+                    store_nt(&psi[base + P::template in_pos<RA>(t, u, r)],
+                             conv<double>(v[k][u * RA + r]));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) P::forward(v[k], t, lds, tw);
+        // A: the forward R-point transform across rows, then w_nx^(i2 k1)
+        xrow_dft_local<R, true>(v);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const size_t base = (M * k + i2) * N;
+#pragma unroll
+            for (int u = 0; u < P::E / RZ; ++u)
+#pragma unroll
+                for (int r = 0; r < RZ; ++r)
+                    store_nt(&work[base + P::template in_pos<RZ>(t, u, r)],
+                             apply(v[k][u * RZ + r], wk[k]));
+        }
+    }
+}
+
 // Launch shape of the x-split row sweep: R lines of Line P per workgroup,
 // lanes line-fast within each quad; the LDS line stride sends the 16 lanes
 // of a group to 16 distinct 16-byte slots (as the line-fast column tiles).
 template <typename T, int N, int V, int R>
 struct XRowLaunch {
     using Line = typename P2Cfg<T, N, V>::Line;
-    static constexpr int THREADS = R * Line::TPL;
+    // rows of 4096 points: the R rows in each thread (row_group_xl; FP32
+    // 8192-point rows spilled more in that layout than one lane per row)
+    static constexpr bool LOCAL = N == 4096;
+    static constexpr int THREADS = LOCAL ? Line::TPL : R * Line::TPL;
     static constexpr int TARGET = 16 / R;
     static constexpr int LSTR =
-        Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
-    static constexpr size_t LDS_BYTES = (size_t)R * LSTR * sizeof(cx<T>);
+        LOCAL ? Line::LS : Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
+    static constexpr size_t LDS_BYTES = (size_t)(LOCAL ? 1 : R) * LSTR * sizeof(cx<T>);
     static constexpr bool OK = THREADS <= 1024 && LDS_BYTES <= 160 * 1024;
+    // The MID sweep prefetches its phase factors (4 registers each) next to
+    // the line elements (4 or 2 registers each) only where those take at
+    // most a quarter of the per-lane register budget of THREADS-thread
+    // workgroups; beyond that the 4096-point groups spilled to scratch.
+    static constexpr int BUDGET = 512 / cmax(1, (THREADS / 64 + 3) / 4);
+    static constexpr int NEED = (LOCAL ? R : 1) * Line::E * (4 + 2 * (int)(sizeof(T) / 4));
+    static constexpr bool EVP = Line::E <= 16 && 4 * NEED <= BUDGET;
 };
 
 template <typename T, int N, int MODE, int V, bool NT, int R>
@@ -334,8 +490,12 @@ __global__ void __launch_bounds__((XRowLaunch<T, N, V, R>::THREADS))
 {
     using XL = XRowLaunch<T, N, V, R>;
     extern __shared__ unsigned char smem_raw[];
-    row_group_x<T, typename XL::Line, R, XL::LSTR, MODE, NT>(
-        jobs[blockIdx.y], blockIdx.x, nx, iev, tw, twx, reinterpret_cast<cx<T> *>(smem_raw));
+    if constexpr (XL::LOCAL)
+        row_group_xl<T, typename XL::Line, R, MODE, NT, XL::EVP>(
+            jobs[blockIdx.y], blockIdx.x, nx, iev, tw, twx, reinterpret_cast<cx<T> *>(smem_raw));
+    else
+        row_group_x<T, typename XL::Line, R, XL::LSTR, MODE, NT, XL::EVP>(
+            jobs[blockIdx.y], blockIdx.x, nx, iev, tw, twx, reinterpret_cast<cx<T> *>(smem_raw));
 }
 
 template <typename T, int N, int MODE, int V, bool NT>
@@ -738,13 +898,15 @@ int launch_1d(const SplitDev *jobs, int njobs, const Prog1D &prog, size_t nsteps
     return set_error(QSIM_ERR_UNSUPPORTED, "no fused kernel for this length");
 
 // Launch variants: lengths 1024 and 2048 have the candidates 0..7, lengths
-// 128, 256 and 512 the two radix plans 0 (16 or 32 elements per thread) and
-// 1 (8 per thread), every other length only variant 0 (P2Cfg).
+// 128, 256, 512 and 4096 the two radix plans 0 (16 or 32 elements per
+// thread) and 1 (8 per thread), every other length only variant 0 (P2Cfg).
 constexpr int kVariants = 8;
 
 int num_variants(size_t n)
 {
-    return (n == 1024 || n == 2048) ? kVariants : (n == 128 || n == 256 || n == 512) ? 2 : 1;
+    return (n == 1024 || n == 2048) ? kVariants
+           : (n == 128 || n == 256 || n == 512 || n == 4096) ? 2
+                                                             : 1;
 }
 bool has_variants(size_t n) { return num_variants(n) > 1; }
 
@@ -759,10 +921,14 @@ bool has_variants(size_t n) { return num_variants(n) > 1; }
 // 25.9 -> 16.1 / 49.0 -> 51.4 us.  (512 FP32 columns in plan 0 batch 13%
 // faster, but their 32 elements per thread would set the register budget of
 // the multi-size group kernels, which mirror these defaults.)
+// 4096-point rows take plan 1 as well (4096^2 x-split rows FP64 / FP32:
+// 187-191 / 128-135 us); 4096-point columns (no x-split) plan 1 in FP64
+// (345 against 516 us) and plan 0 in FP32 (180 against 543 us).
 __host__ __device__ constexpr int default_variant(int prec, size_t n, bool row)
 {
     return n == 1024   ? (prec == 64 ? 4 : 7)
            : n == 2048 ? (prec == 64 ? (row ? 7 : 5) : (row ? 7 : 3))
+           : n == 4096 ? (row || prec == 64 ? 1 : 0)
            : (n == 128 || n == 256 || n == 512) ? 1
                                                 : 0;
 }
@@ -782,7 +948,7 @@ template <typename T, int NN, int MODE, int R>
 int row_vx(int var, const SplitDev *jobs, int njobs, size_t nx, int iev, const void *tw,
            const void *twx, hipStream_t s, bool dry)
 {
-    if constexpr (NN == 128 || NN == 256 || NN == 512)
+    if constexpr (NN == 128 || NN == 256 || NN == 512 || NN == 4096)
         if (var == 1)
             return launch_row_x<T, NN, MODE, 1, R>(jobs, njobs, nx, iev, tw, twx, s, dry);
     if constexpr (NN == 1024 || NN == 2048) {
@@ -804,7 +970,7 @@ int row_v(int var, int R, const SplitDev *jobs, int njobs, size_t nrows, int iev
     if (R == 2) return row_vx<T, NN, MODE, 2>(var, jobs, njobs, nrows, iev, tw, twx, s, dry);
     if (R == 4) return row_vx<T, NN, MODE, 4>(var, jobs, njobs, nrows, iev, tw, twx, s, dry);
     if (dry) return QSIM_OK;
-    if constexpr (NN == 128 || NN == 256 || NN == 512)
+    if constexpr (NN == 128 || NN == 256 || NN == 512 || NN == 4096)
         if (var == 1) return launch_row<T, NN, MODE, 1>(jobs, njobs, nrows, iev, tw, s);
     if constexpr (NN == 1024 || NN == 2048) {
         switch (var) {
@@ -825,7 +991,7 @@ template <typename T, int NN, int R>
 int col_vr(int var, const SplitDev *jobs, int njobs, size_t ncols, int ik, const void *tw,
            hipStream_t s)
 {
-    if constexpr (NN == 128 || NN == 256 || NN == 512)
+    if constexpr (NN == 128 || NN == 256 || NN == 512 || NN == 4096)
         if (var == 1) return launch_col<T, NN, 1, R>(jobs, njobs, ncols, ik, tw, s);
     if constexpr (NN == 1024 || NN == 2048) {
         switch (var) {
@@ -901,7 +1067,7 @@ bool fused_len_ok(size_t n, int prec, int dim)
 // x-split factor a new job starts with.
 int default_xsplit(int prec, size_t nx)
 {
-    return (prec == 64 && nx == 4096) || (prec == 32 && nx == 8192) ? 2 : 1;
+    return nx == 4096 || (prec == 32 && nx == 8192) ? 2 : 1;
 }
 
 // x-split factors a fused 2D job can take: 1, or 2 / 4 when the column
@@ -1207,10 +1373,10 @@ int create(qsim_split_job *j, const double *Vre, const double *Vim)
         QSIM_TRY(twiddle_table(d.nx, d.precision, &j->tw_x));
         if (d.dim == 2) {
             QSIM_TRY(twiddle_table(d.ny, d.precision, &j->tw_y));
-            // columns too long for a tile of several lines in LDS (FP64 4096,
-            // FP32 8192) start x-split in two: measured on MI355X at 4096^2
-            // FP64, the column sweep went from 504 to 178 us and the step
-            // from 699 to 479 us
+            // columns too long for a tile of several lines in LDS (4096, FP32
+            // 8192) start x-split in two: measured on MI355X at 4096^2, the
+            // step went from 548 to 375 us (FP64: columns 345 -> 176 us) and
+            // from 339 to 270 us (FP32-mixed)
             const int R = default_xsplit(d.precision, d.nx);
             if (R != 1 && xsplit_check(j, R, j->row_var) == QSIM_OK)
                 QSIM_TRY(set_xsplit(j, R, j->row_var));

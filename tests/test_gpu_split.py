@@ -396,12 +396,13 @@ def test_graph_replay_is_bitwise_and_its_cache_bounded():
 @pytest.mark.parametrize("shape,method,precision",
                          [((1024, 1024), 2, "fp64"), ((256, 512), 4, "fp64"),
                           ((512, 128), 2, "fp64"), ((2048, 64), 2, "fp64"),
-                          ((1024, 256), 2, "mixed32"), ((256, 2048), 4, "mixed32")])
+                          ((1024, 256), 2, "mixed32"), ((256, 2048), 4, "mixed32"),
+                          ((256, 4096), 2, "fp64"), ((512, 4096), 4, "mixed32")])
 def test_xsplit_schedule_matches_oracle(shape, method, precision, xsplit, oracle):
     """x axis split into R-point transforms across rows (lane exchanges in the
     row sweep) and nx/R-point column transforms: complex128 within 1e-12 of
     the oracle, FP32-mixed within 1e-5, for every row variant the group
-    fits."""
+    fits (rows of 4096 points hold the R rows of a group in each thread)."""
     from qusim_amd import _native as N
     from qusim_amd.api import BoundaryCondition, Evolver2D, Options, SolverMethod
     psi, V, L0, L1 = gauss2d(*shape)
@@ -410,7 +411,8 @@ def test_xsplit_schedule_matches_oracle(shape, method, precision, xsplit, oracle
                          complex_v=True, threads=8)
     norm = np.sqrt(np.sum(np.abs(psi) ** 2) * (L0[1] - L0[0]) / shape[0]
                    * (L1[1] - L1[0]) / shape[1])
-    variants = range(8) if shape[1] in (1024, 2048) else [0]
+    variants = (range(8) if shape[1] in (1024, 2048)
+                else [0, 1] if shape[1] in (128, 256, 512, 4096) else [0])
     for rv in variants:
         o = Options().Hip().Batch(steps).ComplexPotential(True)
         if precision != "fp64":
