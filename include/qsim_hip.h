@@ -332,6 +332,12 @@ int qsim_hip_split_group_advance(qsim_split_group *group, int passes, int use_gr
                                  qsim_stream_t s);
 int qsim_hip_split_group_status(const qsim_split_group *group, const qsim_split_job *job,
                                 int *finished, long long *passes_left);
+/* Member jobs, shape classes, cached captures, and the advances that ran as
+ * direct launches because the runtime refused or invalidated their capture
+ * (graph advances fall back to direct launches, bitwise the same results).
+ * Any out-pointer may be NULL. */
+int qsim_hip_split_group_info(const qsim_split_group *group, int *jobs, int *shapes,
+                              int *graphs, long long *capture_failures);
 int qsim_hip_split_group_destroy(qsim_split_group *group);
 
 /* ------------------------------------------------------ fused Born job

@@ -144,6 +144,8 @@ _SIGS = {
     "qsim_hip_split_group_advance": [_VP, _I, _I, _VP],
     "qsim_hip_split_group_status": [_VP, _VP, ctypes.POINTER(_I),
                                     ctypes.POINTER(ctypes.c_longlong)],
+    "qsim_hip_split_group_info": [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                  ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)],
     "qsim_hip_split_group_destroy": [_VP],
     "qsim_hip_split_get_schedule": [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                     ctypes.POINTER(_I)],

@@ -762,6 +762,7 @@ int mid(int prec, bool inv, const BornDev *tab, int nact, const BornGeom &geo, c
 void destroy(qsim_born_job *j)
 {
     if (!j) return;
+    QSIM_DEVICE_WAIT_GUARD;
     for (void *p : j->owned) (void)hipFree(p);
     if (j->tab_host) (void)hipHostFree(j->tab_host);
     if (j->rtot_host) (void)hipHostFree(j->rtot_host);
@@ -1142,6 +1143,7 @@ int qsim_hip_born_create(qsim_born_job **job, const qsim_born_desc *d, const qsi
 {
     QSIM_CHECK(job && d && V && energies, "null argument");
     *job = nullptr;
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(d->dim == 2 || d->dim == 3, "dim must be 2 or 3");
     QSIM_CHECK(nE >= 1, "need at least one energy");
     QSIM_CHECK(d->precision == 64 || d->precision == 32, "precision must be 64 or 32");

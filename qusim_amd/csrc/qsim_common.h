@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <mutex>
 #include <cstdint>
 #include <string>
 
@@ -183,6 +184,17 @@ int device_cu_count();
 // (kernel, device) for the whole process and under a lock: the attribute is
 // process state, and request loops launch from several threads at once.
 int lds_limit(const void *kernel, size_t bytes);
+// Graph capture and the calls that wait on every stream of the device
+// (hipFree, hipDeviceSynchronize, synchronous copies, null-stream waits,
+// stream creation and destruction) never overlap: a device-wide wait issued
+// by one thread reaches the streams another thread is capturing and
+// invalidates that capture (seen in a request loop whose Born workers
+// created and freed jobs while the main thread captured a group advance).
+// Both sides hold this process-wide lock; it is recursive because entry
+// points nest.
+std::recursive_mutex &device_wait_lock();
+#define QSIM_DEVICE_WAIT_GUARD \
+    std::lock_guard<std::recursive_mutex> qsim_dwg_(qsim::device_wait_lock())
 // A roctx range (rocprofv3 --marker-trace) around an entry point, recorded
 // only when the environment sets QSIM_ROCTX=1; otherwise it costs a branch.
 // Asynchronous entry points mark the host-side enqueue, synchronous ones the

@@ -292,6 +292,7 @@ int qsim_hip_fft_plan_create(qsim_fft_plan **plan, int rank, const size_t *dims,
 {
     QSIM_CHECK(plan, "null plan out-pointer");
     *plan = nullptr;
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(rank >= 1 && rank <= 3, "FFT rank must be 1, 2 or 3");
     QSIM_CHECK(dims, "null dims");
     QSIM_CHECK(precision == 64 || precision == 32, "precision must be 64 or 32");
@@ -340,6 +341,7 @@ int qsim_hip_fft_exec(qsim_fft_plan *p, const void *in, void *out, qsim_stream_t
 
 int qsim_hip_fft_plan_destroy(qsim_fft_plan *p)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     if (p)
         for (int a = 0; a < 3; ++a) release_axis(&p->axis[a]);
     delete p;

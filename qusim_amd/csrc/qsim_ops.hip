@@ -241,6 +241,7 @@ int qsim_hip_set_elem(qsim_c128 *a, size_t idx, qsim_c128 v, qsim_stream_t s)
 int qsim_hip_get_elem(const qsim_c128 *a, size_t idx, qsim_c128 *out, qsim_stream_t s)
 {
     QSIM_CHECK(a && out, "null pointer");
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_HIP(hipStreamSynchronize(as_stream(s)));
     QSIM_HIP(hipMemcpy(out, a + idx, sizeof(qsim_c128), hipMemcpyDeviceToHost));
     return QSIM_OK;
@@ -375,6 +376,7 @@ int qsim_hip_set_elem_real(double *a, size_t idx, double v, qsim_stream_t s)
 int qsim_hip_get_elem_real(const double *a, size_t idx, double *out, qsim_stream_t s)
 {
     QSIM_CHECK(a && out, "null pointer");
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_HIP(hipStreamSynchronize(as_stream(s)));
     QSIM_HIP(hipMemcpy(out, a + idx, sizeof(double), hipMemcpyDeviceToHost));
     return QSIM_OK;

@@ -62,8 +62,15 @@ struct DevScratch {
 static std::mutex g_scratch_mu;
 static DevScratch g_scratch[64];
 
+std::recursive_mutex &device_wait_lock()
+{
+    static std::recursive_mutex mu;
+    return mu;
+}
+
 int scratch_acquire(size_t bytes, Scratch *out)
 {
+    QSIM_DEVICE_WAIT_GUARD;  // before g_scratch_mu: one lock order everywhere
     int d = 0;
     hipError_t e = hipGetDevice(&d);
     if (e != hipSuccess) return hip_error(e, "hipGetDevice");
@@ -139,6 +146,7 @@ int qsim_hip_init(int device)
     QSIM_HIP(hipGetDeviceCount(&n));
     if (n <= 0) return set_error(QSIM_ERR_HIP, "no HIP device visible");
     QSIM_CHECK(device >= 0 && device < n, "qsim_hip_init: device index out of range");
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_HIP(hipSetDevice(device));
     QSIM_HIP(hipFree(nullptr));  // create the context now, not at the first kernel
     return QSIM_OK;
@@ -164,6 +172,11 @@ int qsim_hip_device_info(char *name, size_t name_len, int *cu_count, size_t *mem
 
 int qsim_hip_synchronize(qsim_stream_t stream)
 {
+    if (!stream) {
+        QSIM_DEVICE_WAIT_GUARD;
+        QSIM_HIP(hipStreamSynchronize(nullptr));
+        return QSIM_OK;
+    }
     QSIM_HIP(hipStreamSynchronize(as_stream(stream)));
     return QSIM_OK;
 }
@@ -171,6 +184,7 @@ int qsim_hip_synchronize(qsim_stream_t stream)
 int qsim_hip_stream_create(qsim_stream_t *stream)
 {
     QSIM_CHECK(stream, "null stream out-pointer");
+    QSIM_DEVICE_WAIT_GUARD;
     hipStream_t s;
     QSIM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     *stream = reinterpret_cast<qsim_stream_t>(s);
@@ -179,6 +193,7 @@ int qsim_hip_stream_create(qsim_stream_t *stream)
 
 int qsim_hip_stream_destroy(qsim_stream_t stream)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     if (stream) QSIM_HIP(hipStreamDestroy(as_stream(stream)));
     return QSIM_OK;
 }
@@ -188,6 +203,7 @@ int qsim_hip_malloc(void **ptr, size_t bytes)
     QSIM_CHECK(ptr, "null out-pointer");
     *ptr = nullptr;
     if (bytes == 0) return QSIM_OK;
+    QSIM_DEVICE_WAIT_GUARD;
     hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) {
         *ptr = nullptr;
@@ -200,6 +216,7 @@ int qsim_hip_malloc(void **ptr, size_t bytes)
 
 int qsim_hip_free(void *ptr)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     if (ptr) QSIM_HIP(hipFree(ptr));
     return QSIM_OK;
 }
@@ -208,6 +225,7 @@ int qsim_hip_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
     if (!bytes) return QSIM_OK;
     QSIM_CHECK(dst && src, "null pointer in memcpy_h2d");
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return QSIM_OK;
 }
@@ -216,6 +234,7 @@ int qsim_hip_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
     if (!bytes) return QSIM_OK;
     QSIM_CHECK(dst && src, "null pointer in memcpy_d2h");
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return QSIM_OK;
 }

@@ -1413,6 +1413,7 @@ extern "C" {
 int qsim_hip_split_create(qsim_split_job **job, const qsim_split_desc *d, const double *V_re,
                           const double *V_im, qsim_c128 *psi)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(job && d, "null argument");
     *job = nullptr;
     QSIM_CHECK(d->dim == 1 || d->dim == 2, "dim must be 1 or 2");
@@ -1450,6 +1451,8 @@ int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
     // and replayed (one launch instead of 2*nsteps+1); at most kMaxGraphs
     // step counts are kept per job, the oldest capture is dropped first
     auto it = j->graphs.find(nsteps);
+    std::unique_lock<std::recursive_mutex> lk(device_wait_lock(), std::defer_lock);
+    if (it == j->graphs.end()) lk.lock();  // eviction and capture
     if (it == j->graphs.end() && j->graphs.size() >= kMaxGraphs) {
         // a replay of the graph being dropped may still be in flight
         QSIM_HIP(hipDeviceSynchronize());
@@ -1489,6 +1492,7 @@ int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
 int qsim_hip_split_steps_timed(qsim_split_job *j, size_t nsteps, double *kernel_ms,
                                int *kernel_launches, qsim_stream_t stream)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(j && kernel_ms && kernel_launches, "null argument");
     if (!j->fused)
         return set_error(QSIM_ERR_UNSUPPORTED, "per-kernel timing needs a fused job");
@@ -1512,6 +1516,7 @@ int qsim_hip_split_steps_timed(qsim_split_job *j, size_t nsteps, double *kernel_
 
 int qsim_hip_split_use_graph(qsim_split_job *j, int enable)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(j, "null job");
     j->use_graph = enable != 0;
     if (!j->use_graph) drop_graphs(j);
@@ -1520,6 +1525,7 @@ int qsim_hip_split_use_graph(qsim_split_job *j, int enable)
 
 int qsim_hip_split_set_variants(qsim_split_job *j, int row_variant, int col_variant)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(j, "null job");
     QSIM_CHECK(row_variant >= 0 && row_variant < kVariants && col_variant >= 0 &&
                    col_variant < kVariants,
@@ -1534,6 +1540,7 @@ int qsim_hip_split_set_variants(qsim_split_job *j, int row_variant, int col_vari
 
 int qsim_hip_split_set_xsplit(qsim_split_job *j, int xsplit)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(j, "null job");
     QSIM_TRY(set_xsplit(j, xsplit, j->row_var));
     drop_graphs(j);
@@ -1674,6 +1681,7 @@ extern "C" {
 int qsim_hip_split_autotune(qsim_split_job *j, int reps, int *row_variant, int *col_variant,
                             double *row_us, double *col_us, qsim_stream_t stream)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     qsim::TraceRange tr_("qsim_hip_split_autotune");
     QSIM_CHECK(j, "null job");
     QSIM_CHECK(reps >= 1, "reps must be >= 1");
@@ -1690,6 +1698,7 @@ int qsim_hip_split_autotune_batched(qsim_split_job *const *jobs, int njobs, int 
                                     int *row_variant, int *col_variant, double *row_us,
                                     double *col_us, qsim_stream_t stream)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     qsim::TraceRange tr_("qsim_hip_split_autotune_batched");
     QSIM_CHECK(jobs && njobs >= 1, "null job list");
     QSIM_CHECK(reps >= 1, "reps must be >= 1");
@@ -1713,6 +1722,7 @@ int qsim_hip_split_autotune_batched(qsim_split_job *const *jobs, int njobs, int 
 
 int qsim_hip_split_load_psi(qsim_split_job *j, qsim_stream_t)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     // every qsim_hip_split_steps call starts from the caller's psi
     QSIM_CHECK(j, "null job");
     return QSIM_OK;
@@ -1720,12 +1730,14 @@ int qsim_hip_split_load_psi(qsim_split_job *j, qsim_stream_t)
 
 int qsim_hip_split_norm2(qsim_split_job *j, double *out, qsim_stream_t s)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(j && out, "null argument");
     return qsim_hip_norm2(reinterpret_cast<qsim_c128 *>(j->psi), j->n, out, s);
 }
 
 int qsim_hip_split_destroy(qsim_split_job *j)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     destroy(j);
     return QSIM_OK;
 }
@@ -1733,6 +1745,7 @@ int qsim_hip_split_destroy(qsim_split_job *j)
 int qsim_hip_split_steps_batched(qsim_split_job **jobs, int njobs, size_t nsteps,
                                  qsim_stream_t stream)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     qsim::TraceRange tr_("qsim_hip_split_steps_batched");
     QSIM_CHECK(jobs && njobs >= 0, "null job list");
     if (njobs == 0 || nsteps == 0) return QSIM_OK;
@@ -1782,6 +1795,7 @@ struct qsim_split_batch {
 
 int qsim_hip_split_batch_create(qsim_split_batch **b, int capacity)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(b && capacity > 0, "batch: null out-pointer or capacity < 1");
     *b = nullptr;
     qsim_split_batch *p = new qsim_split_batch();
@@ -1832,6 +1846,7 @@ int qsim_hip_split_batch_steps(qsim_split_batch *b, qsim_split_job *const *jobs,
 
 int qsim_hip_split_batch_destroy(qsim_split_batch *b)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     if (!b) return QSIM_OK;
     if (b->pending) (void)hipEventSynchronize(b->copied);
     if (b->copied) (void)hipEventDestroy(b->copied);
@@ -2095,6 +2110,7 @@ struct qsim_split_group {
     std::vector<hipEvent_t> lane_ev;
     hipEvent_t fork_ev = nullptr;
     std::map<int, hipGraphExec_t> graphs;  // captured advances, by pass count
+    long long capture_failures = 0;        // advances that ran uncaptured instead
 };
 
 namespace {
@@ -2213,10 +2229,10 @@ int group_chain(qsim_split_group *g, int shape, int passes, hipStream_t s)
 // One advance on stream s: the shape chains forked onto the group's lanes
 // (or, under capture, onto parallel branches of the graph) and joined before
 // the pass index moves on.
-int group_enqueue(qsim_split_group *g, int passes, hipStream_t s)
+// One stream and one event per shape chain, created before any capture.
+int group_lanes(qsim_split_group *g)
 {
-    const int ns = (int)g->shapes.size();
-    while ((int)g->lanes.size() < ns) {
+    while (g->lanes.size() < g->shapes.size()) {
         hipStream_t ls;
         hipEvent_t le;
         QSIM_HIP(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
@@ -2224,6 +2240,24 @@ int group_enqueue(qsim_split_group *g, int passes, hipStream_t s)
         QSIM_HIP(hipEventCreateWithFlags(&le, hipEventDisableTiming));
         g->lane_ev.push_back(le);
     }
+    return QSIM_OK;
+}
+
+// Replaces the lane streams and events (after a failed capture, which may
+// leave the streams it forked to in capture mode).
+void group_reset_lanes(qsim_split_group *g)
+{
+    for (hipStream_t ls : g->lanes) (void)hipStreamDestroy(ls);
+    for (hipEvent_t le : g->lane_ev) (void)hipEventDestroy(le);
+    g->lanes.clear();
+    g->lane_ev.clear();
+    (void)hipGetLastError();
+}
+
+int group_enqueue(qsim_split_group *g, int passes, hipStream_t s)
+{
+    const int ns = (int)g->shapes.size();
+    QSIM_TRY(group_lanes(g));
     if (ns == 1) {
         QSIM_TRY(group_chain(g, g->shapes[0], passes, s));
     } else if (ns > 1) {
@@ -2357,6 +2391,7 @@ extern "C" {
 
 int qsim_hip_split_group_destroy(qsim_split_group *g)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     if (!g) return QSIM_OK;
     if (g->pending) (void)hipEventSynchronize(g->copied);
     group_drop_graphs(g);
@@ -2378,6 +2413,7 @@ int qsim_hip_split_group_destroy(qsim_split_group *g)
 
 int qsim_hip_split_group_create(qsim_split_group **out, int capacity, int precision, int method)
 {
+    QSIM_DEVICE_WAIT_GUARD;
     QSIM_CHECK(out && capacity > 0, "group: null out-pointer or capacity < 1");
     QSIM_CHECK(precision == 64 || precision == 32, "group: precision must be 64 or 32");
     QSIM_CHECK(method == 2 || method == 4, "group: method must be 2 (O2) or 4 (O4)");
@@ -2448,6 +2484,10 @@ int qsim_hip_split_group_advance(qsim_split_group *g, int passes, int use_graph,
     qsim::TraceRange tr_("qsim_hip_split_group_advance");
     QSIM_CHECK(g && passes >= 1, "group: null group or passes < 1");
     hipStream_t s = as_stream(stream);
+    // uploads (re)allocate the work lists, captures must not meet another
+    // thread's device-wide wait
+    std::unique_lock<std::recursive_mutex> lk(device_wait_lock(), std::defer_lock);
+    if (g->dirty || (use_graph && !g->graphs.count(passes))) lk.lock();
     if (g->dirty) QSIM_TRY(group_upload(g, s));
     if (!use_graph) {
         QSIM_TRY(group_enqueue(g, passes, s));
@@ -2455,6 +2495,7 @@ int qsim_hip_split_group_advance(qsim_split_group *g, int passes, int use_graph,
         auto it = g->graphs.find(passes);
         if (it == g->graphs.end()) {
             if (g->graphs.size() >= kMaxGraphs) group_drop_graphs(g);
+            QSIM_TRY(group_lanes(g));
             hipStream_t cs;
             QSIM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
             hipGraph_t gr = nullptr;
@@ -2474,12 +2515,32 @@ int qsim_hip_split_group_advance(qsim_split_group *g, int passes, int use_graph,
             }
             if (gr) (void)hipGraphDestroy(gr);
             (void)hipStreamDestroy(cs);
-            if (rc != QSIM_OK) return rc;
+            if (rc != QSIM_OK) {
+                // the runtime refused or invalidated the capture: this advance
+                // runs as direct launches (the same kernels, bitwise the same
+                // results) on fresh lanes, and the next one captures again
+                group_reset_lanes(g);
+                ++g->capture_failures;
+                QSIM_TRY(group_enqueue(g, passes, s));
+                g->base += passes;
+                return QSIM_OK;
+            }
             it = g->graphs.emplace(passes, ex).first;
         }
         QSIM_HIP(hipGraphLaunch(it->second, s));
     }
     g->base += passes;
+    return QSIM_OK;
+}
+
+int qsim_hip_split_group_info(const qsim_split_group *g, int *jobs, int *shapes, int *graphs,
+                              long long *capture_failures)
+{
+    QSIM_CHECK(g, "group: null argument");
+    if (jobs) *jobs = (int)g->jobs.size();
+    if (shapes) *shapes = (int)g->shapes.size();
+    if (graphs) *graphs = (int)g->graphs.size();
+    if (capture_failures) *capture_failures = g->capture_failures;
     return QSIM_OK;
 }
 

@@ -412,9 +412,17 @@ class _GroupLane:
         self.jobs = keep
         return done
 
+    def capture_failures(self):
+        """advances that ran as direct launches because their graph capture
+        was refused (the same kernels and results)"""
+        n = ctypes.c_longlong()
+        N.call("qsim_hip_split_group_info", self.group, None, None, None, ctypes.byref(n))
+        return n.value
+
     def close(self):
         if self.inflight is not None:
             self.event.synchronize()
+        self.failed_captures = self.capture_failures()
         N.call("qsim_hip_split_group_destroy", self.group)
         N.call("qsim_hip_stream_destroy", self.stream)
 
@@ -448,6 +456,7 @@ class Server:
         self._graveyard = []
         # an exception that ended a Born worker thread
         self._worker_error = None
+        self.capture_failures = 0
 
     # footprint accounting (shared with the Born worker)
     def _reserve(self, fp):
@@ -615,6 +624,7 @@ class Server:
                 w.join()
             for lane in lanes.values():
                 lane.close()
+                self.capture_failures += getattr(lane, "failed_captures", 0)
             self._flush()
             staging.close()
         self.makespan = time.perf_counter() - t0
@@ -645,7 +655,7 @@ class Server:
             "per_kind": {k: {"count": len(lat[k]), "p50_s": pct(lat[k], 50),
                              "p99_s": pct(lat[k], 99)} for k in KINDS},
             "peak_footprint_bytes": self.peak_used, "hbm_budget_bytes": self.budget,
-            "precision": self.precision,
+            "precision": self.precision, "graph_capture_failures": self.capture_failures,
         }
 
 

@@ -72,6 +72,11 @@ def test_group_of_uneven_sizes_equals_each_job_alone(precision, method, use_grap
                     done.add(i)
             assert adv < 100
         N.call("qsim_hip_synchronize", None)
+        nj, ns, ng, nf = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong())
+        N.call("qsim_hip_split_group_info", g, ctypes.byref(nj), ctypes.byref(ns),
+               ctypes.byref(ng), ctypes.byref(nf))
+        assert nj.value == 0 and nf.value == 0
+        assert (ng.value >= 1) == bool(use_graph)
         for i, (shape, steps, _) in enumerate(plan):
             N.call("qsim_hip_split_steps", alone[i][0]._job, steps, None)
             got, want = jobs[i][0].GetPsi(), alone[i][0].GetPsi()

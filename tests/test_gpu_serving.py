@@ -125,6 +125,9 @@ def test_full_size_mix_spot_checks():
     st = s.run(reqs)
     print({k: st[k] for k in ("completed", "solves_per_s", "latency_p50_s", "latency_p99_s")})
     assert st["completed"] == len(reqs) and st["errors"] == 0
+    # the Born workers create and free jobs while the main thread captures
+    # group advances: no capture may be invalidated by them
+    assert st["graph_capture_failures"] == 0
     sc = spot_check(reqs, s.results, every=50)
     print(sc)
     assert sum(v["checked"] for v in sc.values()) == len(keep)
