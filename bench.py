@@ -146,37 +146,42 @@ def host_threads(cap=16):
 
 
 def traffic_from_profiles(name, schedule):
-    """HBM bytes per launch of `name` from the committed PMC summary, when it
-    was measured with this launch schedule (row, column, x-split); else None."""
+    """HBM bytes per launch of sweep `name` ("row_sweep" or "col_sweep") under
+    the launch schedule (row variant, column variant, x-split) from the
+    committed PMC summary, when that kernel was measured there; else None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
+        import re
         t = json.load(open(path))
-        if list(t.get("schedule", [])) != list(schedule):
-            return None
-        ent = t["kernels"].get(name)
-        return None if ent is None else ent["bytes_per_launch"]
+        pat = re.compile(dict(probe_kernels(*schedule))[name])
+        for kname, ent in t["by_kernel"].items():
+            if pat.search(kname):
+                return ent["bytes_per_launch"]
+        return None
     except Exception:
         return None
 
 
 # kernels the PMC summary reports: (key, substring of the rocprof kernel name)
 PROBE_COPY_N = 1 << 20          # complex128 elements per calibration copy (16 MiB each way)
-# launch schedule of the traffic probe (row variant, column variant, x-split
-# factor): the one the autotuner picks for the C2 job on MI355X
-PROBE_SCHEDULE = (6, 0, 2)
+# launch schedules of the traffic probe (row variant, column variant, x-split
+# factor): the ones the autotuner picks for the C2 job on MI355X
+PROBE_SCHEDULES = ((6, 0, 2), (6, 1, 2), (6, 4, 1), (6, 5, 1), (6, 1, 1), (0, 0, 4))
 
 
 def probe_kernels(rv, cv, xs):
-    """(key, substring of the rocprof kernel name) of the probe's sweeps"""
+    """(key, regular expression of the rocprof kernel name) of the C2 middle
+    row sweep and column sweep under a launch schedule.  x-split row sweeps
+    are instantiated per radix plan: row variants 1 and 5 run plan 0, 3 runs
+    2, 7 runs 6."""
     if xs == 1:
-        return (("copy", "qsim_hip_copy"),
-                ("row_sweep", "k_row_sweep<double, 1024, 1, %d," % rv),
-                ("col_sweep", "k_col_sweep<double, 1024, %d," % cv))
-    return (("copy", "qsim_hip_copy"),
-            ("row_sweep", "k_row_sweep_x<double, 1024, 1, %d," % rv),
-            ("col_sweep", "k_col_sweep<double, %d, %d," % (1024 // xs, cv)))
+        return (("row_sweep", r"k_row_sweep<double, 1024, 1, %d, (true|false)>" % rv),
+                ("col_sweep", r"k_col_sweep<double, 1024, %d, (true|false), 1>" % cv))
+    plan = {1: 0, 5: 0, 3: 2, 7: 6}.get(rv, rv)
+    return (("row_sweep", r"k_row_sweep_x<double, 1024, 1, %d, (true|false), %d>" % (plan, xs)),
+            ("col_sweep", r"k_col_sweep<double, %d, %d, (true|false), %d>" % (1024 // xs, cv, xs)))
 
 
 def batch_probe(steps, njobs, variants, n=1024):
@@ -236,14 +241,15 @@ def traffic_probe(steps):
         Nn.call("qsim_hip_copy", b.ptr, a.ptr, PROBE_COPY_N, None)
     Nn.synchronize()
     job, keep = make_job(Nn, c2_problem(), 64)
-    rv, cv, xs = PROBE_SCHEDULE
-    Nn.call("qsim_hip_split_set_xsplit", job, xs)
-    Nn.call("qsim_hip_split_set_variants", job, rv, cv)
-    Nn.call("qsim_hip_split_steps", job, steps, None)
-    Nn.synchronize()
+    for rv, cv, xs in PROBE_SCHEDULES:
+        Nn.call("qsim_hip_split_set_variants", job, 0, 0)
+        Nn.call("qsim_hip_split_set_xsplit", job, xs)
+        Nn.call("qsim_hip_split_set_variants", job, rv, cv)
+        Nn.call("qsim_hip_split_steps", job, steps, None)
+        Nn.synchronize()
     Nn.call("qsim_hip_split_destroy", job)
-    log("traffic probe: 3 copies of %d complex128, %d C2 steps, schedule %s"
-        % (PROBE_COPY_N, steps, PROBE_SCHEDULE))
+    log("traffic probe: 3 copies of %d complex128, %d C2 steps under each schedule %s"
+        % (PROBE_COPY_N, steps, PROBE_SCHEDULES))
 
 
 def _pmc_rows(d):
@@ -273,27 +279,36 @@ def _pmc_avg(rows, counter, sub):
 def pmc_summary(fetch_dir, write_dir):
     """Turn the FETCH_SIZE and WRITE_SIZE passes into profiles/traffic.json:
     each counter (KiB per dispatch) scaled by what it reads for the copy
-    kernel of known size."""
+    kernel of known size, for every middle row sweep and column sweep of the
+    probe's schedules (by kernel name)."""
+    import re
     fr, wr = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
     copy_kib = PROBE_COPY_N * 16 / 1024.0
-    kern = probe_kernels(*PROBE_SCHEDULE)
-    cf, nf = _pmc_avg(fr, "FETCH_SIZE", kern[0][1])
-    cw, nw = _pmc_avg(wr, "WRITE_SIZE", kern[0][1])
+    cf, nf = _pmc_avg(fr, "FETCH_SIZE", "qsim_hip_copy")
+    cw, nw = _pmc_avg(wr, "WRITE_SIZE", "qsim_hip_copy")
     out = {"calibration": {"copy_bytes_each_way": copy_kib * 1024, "copy_fetch_kib": cf,
                            "copy_write_kib": cw, "fetch_factor": copy_kib / cf,
                            "write_factor": copy_kib / cw, "dispatches": [nf, nw]},
-           "schedule": list(PROBE_SCHEDULE), "kernels": {}}
-    alg = {"row_sweep": ROW_B_PER_PT[64] * (1 << 20), "col_sweep": COL_B_PER_PT[64] * (1 << 20)}
-    for key, sub in kern[1:]:
-        f, n1 = _pmc_avg(fr, "FETCH_SIZE", sub)
-        w, n2 = _pmc_avg(wr, "WRITE_SIZE", sub)
+           "schedules": [list(x) for x in PROBE_SCHEDULES], "by_kernel": {}}
+    names = sorted({re.sub(r"\(.*", "", r["Kernel_Name"]).replace("void ", "") for r in fr}
+                   | {re.sub(r"\(.*", "", r["Kernel_Name"]).replace("void ", "") for r in wr})
+    for kname in names:
+        if re.search(r"k_row_sweep(_x)?<double, 1024, 1,", kname):
+            alg = ROW_B_PER_PT[64] * (1 << 20)
+        elif "k_col_sweep<double" in kname:
+            alg = COL_B_PER_PT[64] * (1 << 20)
+        else:
+            continue
+        f, n1 = _pmc_avg(fr, "FETCH_SIZE", kname + "(")
+        w, n2 = _pmc_avg(wr, "WRITE_SIZE", kname + "(")
         if f is None or w is None:
             continue
         fb = f * 1024 * out["calibration"]["fetch_factor"]
         wb = w * 1024 * out["calibration"]["write_factor"]
-        out["kernels"][key] = {"fetch_bytes": fb, "write_bytes": wb, "bytes_per_launch": fb + wb,
-                               "algorithmic_bytes": alg[key], "ratio": (fb + wb) / alg[key],
-                               "raw_fetch_kib": f, "raw_write_kib": w, "dispatches": [n1, n2]}
+        out["by_kernel"][kname] = {"fetch_bytes": fb, "write_bytes": wb,
+                                   "bytes_per_launch": fb + wb, "algorithmic_bytes": alg,
+                                   "ratio": (fb + wb) / alg, "raw_fetch_kib": f,
+                                   "raw_write_kib": w, "dispatches": [n1, n2]}
     path = os.path.join(ROOT, "profiles", "traffic.json")
     json.dump(out, open(path, "w"), indent=1)
     print(json.dumps(out, indent=1))
