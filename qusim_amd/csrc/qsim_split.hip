@@ -733,6 +733,8 @@ struct qsim_split_job {
     // graph_order)
     bool use_graph = false;
     std::map<size_t, hipGraphExec_t> graphs;
+    hipEvent_t graph_done = nullptr;  // follows the job's last replay
+    bool replayed = false;
     std::vector<size_t> graph_order;
     // phase program
     int ev_start = 0, ev_end = 0, nph = 0;
@@ -1229,13 +1231,18 @@ int unfused_steps(qsim_split_job *j, size_t nsteps, hipStream_t s)
     return QSIM_OK;
 }
 
-// Captured graphs may still be replaying on some stream: wait for the device
-// before destroying them (only when the job has any, so jobs that never use
-// graphs never wait here).
+// Captured graphs may still be replaying: wait for the job's last replay
+// (not for the whole device) before destroying them.
+void wait_replays(qsim_split_job *j)
+{
+    if (j->replayed) (void)hipEventSynchronize(j->graph_done);
+    j->replayed = false;
+}
+
 void drop_graphs(qsim_split_job *j)
 {
     if (j->graphs.empty()) return;
-    (void)hipDeviceSynchronize();
+    wait_replays(j);
     for (auto &kv : j->graphs) (void)hipGraphExecDestroy(kv.second);
     j->graphs.clear();
     j->graph_order.clear();
@@ -1245,6 +1252,7 @@ void destroy(qsim_split_job *j)
 {
     if (!j) return;
     drop_graphs(j);
+    if (j->graph_done) (void)hipEventDestroy(j->graph_done);
     for (void *p : j->owned) (void)hipFree(p);
     if (j->fwd) qsim_hip_fft_plan_destroy(j->fwd);
     if (j->inv) qsim_hip_fft_plan_destroy(j->inv);
@@ -1455,13 +1463,15 @@ int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
     if (it == j->graphs.end()) lk.lock();  // eviction and capture
     if (it == j->graphs.end() && j->graphs.size() >= kMaxGraphs) {
         // a replay of the graph being dropped may still be in flight
-        QSIM_HIP(hipDeviceSynchronize());
+        wait_replays(j);
         auto old = j->graphs.find(j->graph_order.front());
         (void)hipGraphExecDestroy(old->second);
         j->graphs.erase(old);
         j->graph_order.erase(j->graph_order.begin());
     }
     if (it == j->graphs.end()) {
+        if (!j->graph_done)
+            QSIM_HIP(hipEventCreateWithFlags(&j->graph_done, hipEventDisableTiming));
         hipStream_t cs;
         QSIM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         hipGraph_t g = nullptr;
@@ -1485,7 +1495,12 @@ int qsim_hip_split_steps(qsim_split_job *j, size_t nsteps, qsim_stream_t stream)
         it = j->graphs.emplace(nsteps, ex).first;
         j->graph_order.push_back(nsteps);
     }
+    // replays on another stream than the last one follow it, so graph_done
+    // always covers every replay in flight
+    if (j->replayed) QSIM_HIP(hipStreamWaitEvent(s, j->graph_done, 0));
     QSIM_HIP(hipGraphLaunch(it->second, s));
+    QSIM_HIP(hipEventRecord(j->graph_done, s));
+    j->replayed = true;
     return QSIM_OK;
 }
 
@@ -2109,18 +2124,50 @@ struct qsim_split_group {
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> lane_ev;
     hipEvent_t fork_ev = nullptr;
-    std::map<int, hipGraphExec_t> graphs;  // captured advances, by pass count
-    long long capture_failures = 0;        // advances that ran uncaptured instead
+    // non-temporal line loads of each shape's row and column kernels, chosen
+    // with the grids
+    bool nt_row[kGroupShapes] = {}, nt_col[kGroupShapes] = {};
+    // captured advances, keyed by their launch configuration (pass count and,
+    // per shape, grids and load form), oldest first in graph_order; graph_done
+    // follows the group's last replay
+    std::map<std::vector<long long>, hipGraphExec_t> graphs;
+    std::vector<std::vector<long long>> graph_order;
+    hipEvent_t graph_done = nullptr;
+    bool replayed = false;
+    long long captures = 0;
+    long long capture_failures = 0;  // advances that ran uncaptured instead
 };
 
 namespace {
 
+// A captured advance may still be replaying on the group's stream: wait for
+// the last replay (not for the whole device) before destroying captures.
+void group_wait_replays(qsim_split_group *g)
+{
+    if (g->replayed) (void)hipEventSynchronize(g->graph_done);
+    g->replayed = false;
+}
+
 void group_drop_graphs(qsim_split_group *g)
 {
     if (g->graphs.empty()) return;
-    (void)hipDeviceSynchronize();
+    group_wait_replays(g);
     for (auto &kv : g->graphs) (void)hipGraphExecDestroy(kv.second);
     g->graphs.clear();
+    g->graph_order.clear();
+}
+
+// The launch configuration a captured advance of `passes` passes holds.
+std::vector<long long> group_graph_key(const qsim_split_group *g, int passes)
+{
+    std::vector<long long> k{passes};
+    for (int sh : g->shapes) {
+        k.push_back(sh);
+        k.push_back(g->grid_row[sh]);
+        k.push_back(g->grid_col[sh]);
+        k.push_back(g->nt_row[sh] * 2 + g->nt_col[sh]);
+    }
+    return k;
 }
 
 // lines per workgroup and LDS bytes of one length's group kernels
@@ -2171,16 +2218,12 @@ int group_chain_t(qsim_split_group *g, int shape, int passes, hipStream_t s)
 {
     using GR = GroupLaunch<T, NY, false>;
     using GC = GroupLaunch<T, NX, true>;
-    // non-temporal line loads by the rule of the standalone sweeps (lines of
-    // up to 1024 points, 8 MiB or more per launch), on the shape's lines when
-    // the chain was set up
-    const size_t bytes = g->shape_points[shape] * sizeof(cx<T>);
     auto kr = k_row_group<T, NY, false>;
     auto kc = k_col_group<T, NX, false>;
     if constexpr (NY <= 1024)
-        if (line_nt(NY, bytes)) kr = k_row_group<T, NY, true>;
+        if (g->nt_row[shape]) kr = k_row_group<T, NY, true>;
     if constexpr (NX <= 1024)
-        if (line_nt(NX, bytes)) kc = k_col_group<T, NX, true>;
+        if (g->nt_col[shape]) kc = k_col_group<T, NX, true>;
     QSIM_TRY(lds_limit((const void *)kr, GR::LDS_BYTES));
     QSIM_TRY(lds_limit((const void *)kc, GC::LDS_BYTES));
     for (int r = 0; r < passes; ++r) {
@@ -2299,17 +2342,26 @@ int group_upload(qsim_split_group *g, hipStream_t s)
         need += nrow + ncol_of[sh];
     }
     if (shapes != g->shapes) {
-        group_drop_graphs(g);
+        // grids in powers of two (at least 8 workgroups, at most the group's
+        // grid), so a shape set that comes back finds its captures again
         g->shapes = shapes;
         auto fit = [g](size_t n) {
-            const size_t r = std::min<size_t>((n + 7) / 8 * 8, (size_t)g->grid);
-            return (int)std::max<size_t>(r, 8);
+            size_t r = 8;
+            while (r < n && r < (size_t)g->grid) r *= 2;
+            return (int)std::min<size_t>(r, (size_t)g->grid);
         };
+        const size_t esz = g->prec == 64 ? sizeof(c128) : sizeof(c64);
         for (int sh : shapes) {
             g->grid_row[sh] = fit(nrow_of[sh]);
             g->grid_col[sh] = fit(ncol_of[sh]);
             g->shape_points[sh] = 0;
             for (int i : by[sh]) g->shape_points[sh] += g->jobs[i]->n;
+            // non-temporal line loads by the rule of the standalone sweeps
+            // (lines of up to 1024 points, 8 MiB or more per launch)
+            const qsim_split_job *j0 = g->jobs[by[sh][0]];
+            const size_t bytes = g->shape_points[sh] * esz;
+            g->nt_row[sh] = j0->d.ny <= 1024 && line_nt(j0->d.ny, bytes);
+            g->nt_col[sh] = j0->d.nx <= 1024 && line_nt(j0->d.nx, bytes);
         }
     }
     if (g->pending) QSIM_HIP(hipEventSynchronize(g->copied));
@@ -2399,6 +2451,7 @@ int qsim_hip_split_group_destroy(qsim_split_group *g)
     for (hipStream_t ls : g->lanes) (void)hipStreamDestroy(ls);
     for (hipEvent_t le : g->lane_ev) (void)hipEventDestroy(le);
     if (g->fork_ev) (void)hipEventDestroy(g->fork_ev);
+    if (g->graph_done) (void)hipEventDestroy(g->graph_done);
     if (g->copied) (void)hipEventDestroy(g->copied);
     if (g->djobs) (void)hipFree(g->djobs);
     if (g->hjobs) (void)hipHostFree(g->hjobs);
@@ -2434,6 +2487,7 @@ int qsim_hip_split_group_create(qsim_split_group **out, int capacity, int precis
         hipMalloc(&g->dbase, sizeof(long long)) != hipSuccess ||
         hipEventCreateWithFlags(&g->copied, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&g->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->graph_done, hipEventDisableTiming) != hipSuccess ||
         hipMemcpy(g->dbase, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipGetLastError();
         rc = set_error(QSIM_ERR_NOMEM, "group: allocation failed");
@@ -2487,14 +2541,24 @@ int qsim_hip_split_group_advance(qsim_split_group *g, int passes, int use_graph,
     // uploads (re)allocate the work lists, captures must not meet another
     // thread's device-wide wait
     std::unique_lock<std::recursive_mutex> lk(device_wait_lock(), std::defer_lock);
-    if (g->dirty || (use_graph && !g->graphs.count(passes))) lk.lock();
-    if (g->dirty) QSIM_TRY(group_upload(g, s));
+    if (g->dirty) {
+        lk.lock();
+        QSIM_TRY(group_upload(g, s));
+    }
     if (!use_graph) {
         QSIM_TRY(group_enqueue(g, passes, s));
     } else {
-        auto it = g->graphs.find(passes);
+        const std::vector<long long> key = group_graph_key(g, passes);
+        auto it = g->graphs.find(key);
         if (it == g->graphs.end()) {
-            if (g->graphs.size() >= kMaxGraphs) group_drop_graphs(g);
+            if (!lk.owns_lock()) lk.lock();
+            if (g->graphs.size() >= kMaxGraphs) {
+                group_wait_replays(g);
+                auto old = g->graphs.find(g->graph_order.front());
+                (void)hipGraphExecDestroy(old->second);
+                g->graphs.erase(old);
+                g->graph_order.erase(g->graph_order.begin());
+            }
             QSIM_TRY(group_lanes(g));
             hipStream_t cs;
             QSIM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -2525,9 +2589,13 @@ int qsim_hip_split_group_advance(qsim_split_group *g, int passes, int use_graph,
                 g->base += passes;
                 return QSIM_OK;
             }
-            it = g->graphs.emplace(passes, ex).first;
+            it = g->graphs.emplace(key, ex).first;
+            g->graph_order.push_back(key);
+            ++g->captures;
         }
         QSIM_HIP(hipGraphLaunch(it->second, s));
+        QSIM_HIP(hipEventRecord(g->graph_done, s));
+        g->replayed = true;
     }
     g->base += passes;
     return QSIM_OK;

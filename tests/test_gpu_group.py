@@ -119,3 +119,77 @@ def test_group_refuses_what_it_cannot_run():
         assert fin and left == 0
     finally:
         N.call("qsim_hip_split_group_destroy", g)
+
+
+def _info(g):
+    from qusim_amd import _native as N
+    v = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()]
+    N.call("qsim_hip_split_group_info", g, *[ctypes.byref(x) for x in v])
+    return [x.value for x in v]
+
+
+def test_group_captures_follow_the_shape_set():
+    """Captured advances are cached by their launch configuration: when a
+    shape set comes back (a 512^2 job joins a 256^2 one and leaves again), the
+    group replays its earlier capture instead of capturing again, and the
+    results stay bitwise those of direct launches."""
+    from qusim_amd import _native as N
+    groups, jobs = [], []
+    for use_graph in (1, 0):
+        g = ctypes.c_void_p()
+        N.call("qsim_hip_split_group_create", ctypes.byref(g), 4, 32, 2)
+        groups.append(g.value)
+        jobs.append((_job((256, 256), 5, "mixed32", 2)[0], _job((512, 512), 6, "mixed32", 2)[0]))
+    try:
+        seen = []
+        for gi, use_graph in enumerate((1, 0)):
+            g, (a, b) = groups[gi], jobs[gi]
+            N.call("qsim_hip_split_group_add", g, a._job, 40)
+            N.call("qsim_hip_split_group_advance", g, 5, use_graph, None)
+            N.call("qsim_hip_split_group_add", g, b._job, 6)
+            adv = 0
+            while not _status(g, b._job)[0]:
+                N.call("qsim_hip_split_group_advance", g, 5, use_graph, None)
+                adv += 1
+                assert adv < 20
+            N.call("qsim_hip_split_group_remove", g, b._job)
+            while not _status(g, a._job)[0]:
+                N.call("qsim_hip_split_group_advance", g, 5, use_graph, None)
+                adv += 1
+                assert adv < 40
+            N.call("qsim_hip_synchronize", None)
+            seen.append(_info(g))
+        # the graph group holds two captures ({256^2} and {256^2, 512^2}),
+        # none failed; the direct group none
+        assert seen[0][2] == 2 and seen[0][3] == 0, seen
+        assert seen[1][2] == 0, seen
+        assert np.array_equal(jobs[0][0].GetPsi(), jobs[1][0].GetPsi())
+        assert np.array_equal(jobs[0][1].GetPsi(), jobs[1][1].GetPsi())
+    finally:
+        for g in groups:
+            N.call("qsim_hip_split_group_destroy", g)
+
+
+def test_job_graph_replays_on_two_streams():
+    """A job's captured steps replayed on alternating streams give the bits
+    of direct launches (each replay follows the last one)."""
+    from qusim_amd import _native as N
+    a, _, _ = _job((256, 256), 7, "fp64", 2)
+    b, _, _ = _job((256, 256), 7, "fp64", 2)
+    streams = []
+    for _ in range(2):
+        p = ctypes.c_void_p()
+        N.call("qsim_hip_stream_create", ctypes.byref(p))
+        streams.append(p.value)
+    try:
+        N.call("qsim_hip_split_use_graph", b._job, 1)
+        for i in range(6):
+            N.call("qsim_hip_split_steps", a._job, 3, None)
+            N.call("qsim_hip_split_steps", b._job, 3, streams[i % 2])
+        N.call("qsim_hip_synchronize", None)
+        for st in streams:
+            N.call("qsim_hip_synchronize", st)
+        assert np.array_equal(a.GetPsi(), b.GetPsi())
+    finally:
+        for st in streams:
+            N.call("qsim_hip_stream_destroy", st)
