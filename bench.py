@@ -686,7 +686,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=10,
+    ap.add_argument("--cpu-steps", type=int, default=50,
                     help="C2 steps per timed CPU run (seq and par; strong runs 5x)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")

@@ -15,12 +15,29 @@
  *                    form; norm conservation; tunnelling probability
  *                    0.15937744085 (applications/QuEvol1DPrecision/
  *                    QuSimPrecision.cpp:82); packet velocity 1 (:217-218).
+ *     complex V      golden trajectories of the compiled reference's complex
+ *                    Exp / MulExp device ops (src/DeviceTemplate.h:1294-1318,
+ *                    1468-1496) with an imaginary absorbing layer, 1D and 2D,
+ *                    O2 and O4; constant V = -iW gives e^{-WT} times the free
+ *                    evolution; the norm never grows when Im V <= 0.
+ *     fnes           golden trajectories of Step() with FNES; equals the
+ *                    plain run renormalised after every call.
+ *   orc_observables  golden values of the compiled reference's Abs2K1D /
+ *                    Abs2K2D / Abs2Idx2 / Abs2Mul / Abs2Idx / Norm2 ops;
+ *                    plane-wave and Gaussian-packet closed forms of <T>, <V>,
+ *                    <x>.
  *   orc_born_run     golden iterates from the compiled reference
  *                    (Perburbation.h BornSeries / PreconditionalBornSeries);
  *                    first-Born closed forms (applications/
  *                    QuScattering{2,3}DPrecision).
  *   orc_xsection*    golden values from the compiled reference Xsection3D;
  *                    first-Born closed forms.
+ *   orc_total_xsection*  the first-Born closed form of a weak Gaussian summed
+ *                    by the same angular rule, and against the exact integral
+ *                    (2D: e^{-x} I0(x); 3D: the reference's weight 4 pi /
+ *                    (ntheta npsi) at src/ScatteringSolverImpl.cpp:377 makes
+ *                    the total 2 / pi times the integral).
+ *   orc_split2d_time, orc_born_time  the run loops above, timed.
  */
 #include "qsim_oracle.h"
 
