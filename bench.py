@@ -587,11 +587,13 @@ def fp32_section(steps_err, steps_rate, fp64_plain_1024, oracle_threads):
 # Algorithmic bytes per point-iteration of a Born iteration: a pencil sweep
 # (x lines: read + write of one complex array) and one fused y-z plane sweep
 # reading tmp, dpsi and V and writing dpsi and tmp, with psi0 generated on the
-# fly: 16 + 40 = 56 B in complex64 (112 B in complex128).  The schedule this
-# library runs for FP32 128^2 planes spends 72 B: plane END 32 (tmp, dpsi,
-# complex64 V read; dpsi written) + plane START 24 + outer x pass 16.
+# fly: 16 + 40 = 56 B in complex64 (112 B in complex128).  For FP32 128^2
+# planes with the Vellekoop, Hao1 or identity preconditioner (C4 uses
+# Vellekoop) the library runs exactly that schedule (one merged y-z plane
+# pass + the outer x pass); complex128 runs the inner, two middle and the
+# outer passes (176 B).
 BORN_B_ALG = {64: 112.0, 32: 56.0}
-BORN_B_SCHEDULE = {64: 176.0, 32: 72.0}
+BORN_B_SCHEDULE = {64: 176.0, 32: 56.0}
 
 
 def born_section(max_iter=200, tol=1e-8, checks=(0, 7, 31)):

@@ -276,11 +276,12 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
 // y-z plane sweep (3D, FP32 iterates, n1 = n2 = 128): one workgroup per
 // (x plane, energy) holds the contiguous 128 x 128 plane.  It replaces the
 // middle-axis passes and the inner sweep: END runs IFFT_y, IFFT_z, the
-// preconditioned update and residual sums; START runs Add3, FFT_z, FFT_y.
-// With a complex64 copy of V an iteration moves 72 B per point (END 32,
-// START 24, outer 16) instead of 96.  y lines are strided in the plane: they are loaded line-fast (lane ->
-// z position), so every load and store is coalesced; the switch between
-// y lines and z lines is a transpose through LDS.
+// preconditioned update and residual sums; START runs Add3, FFT_z, FFT_y;
+// BOTH runs END then START in one pass.  With a complex64 copy of V an
+// iteration moves 56 B per point (BOTH 40, outer 16) instead of 96 (72 with
+// END and START apart).  y lines are strided in the plane: they are loaded
+// line-fast (lane -> z position), so every load and store is coalesced; the
+// switch between y lines and z lines is a transpose through LDS.
 constexpr int PLANE_N = 128;
 using PlaneLine = Pow2Line<float, PLANE_N, 16, 8, 1>;   // E = 16, 8 threads per line
 constexpr int PLANE_THREADS = PLANE_N * PlaneLine::TPL;   // 1024
@@ -294,7 +295,13 @@ constexpr size_t PLANE_LDS =
 // workgroup.  With the recurrence form of the twiddle powers the END pass
 // spilled 28 B per lane (Vellekoop, Hao1); the product-tree form leaves it
 // without spills (C4: +2.5%).  Hao2's END spills more with the tree (76 B
-// against 12 B), so it keeps the recurrence.
+// against 12 B), so it keeps the recurrence.  BOTH fits the same budget
+// (Vellekoop, Hao1: 56 B per lane of scratch, against 220-350 B) because
+// (a) it reads each point's dpsi in the update loop instead of prefetching
+// the line, (b) compiler barriers every two points keep the update's loads
+// from being hoisted all at once, and (c) START takes its lane coordinates
+// from an opaque copy of the thread index, so END's address offsets are
+// recomputed rather than held across the pass.
 template <int PRE>
 using PlaneLineFor = Pow2Line<float, PLANE_N, 16, 8, 1, 1, PRE != PRE_HAO2>;
 
@@ -324,11 +331,15 @@ __global__ void __launch_bounds__(PLANE_THREADS)
     // prefetching both would spill at 128 registers).  dpsi and tmp are this
     // energy's and stream through once (non-temporal loads when the launch
     // holds several energies, NT); the complex64 V is read by every energy
-    // and keeps normal loads, so it stays in L2.
-    c64 dd[P::E];
+    // and keeps normal loads, so it stays in L2.  BOTH reads dpsi in the
+    // update loop instead (a).
+    constexpr bool PRE_DD = MODE != BA_BOTH;
+    c64 dd[PRE_DD ? P::E : 1];
+    if constexpr (PRE_DD) {
 #pragma unroll
-    for (int r = 0; r < RA; ++r)
-        dd[r] = ld_stream<NT>(dp + pbase + (size_t)j * N + P::template in_pos<RA>(t, 0, r));
+        for (int r = 0; r < RA; ++r)
+            dd[r] = ld_stream<NT>(dp + pbase + (size_t)j * N + P::template in_pos<RA>(t, 0, r));
+    }
     c64 v[P::E];
     if constexpr (MODE != BA_START) {
         // y lines, inverse input order
@@ -351,6 +362,7 @@ __global__ void __launch_bounds__(PLANE_THREADS)
         __syncthreads();
         P::inverse(v, t, lds + j * PLANE_LS, tw);
     }
+    if constexpr (MODE == BA_BOTH) asm volatile("" ::: "memory");  // (b)
     const double ab_line = ld_g(J.ab[0] + i) + ld_g(J.ab[1] + j);
     const c128 pw_line = ld_g(J.pw[0] + i) * ld_g(J.pw[1] + j);
     double rsum = 0, nsum = 0;
@@ -359,7 +371,9 @@ __global__ void __launch_bounds__(PLANE_THREADS)
         const int k = P::template in_pos<RA>(t, 0, r);
         const c128 vk = cvt<double>(ld_g(Vf + pbase + (size_t)j * N + k));
         const c128 ve = PRE == PRE_NAIVE ? vk : mk(vk.x, vk.y + (ab_line + ld_g(J.ab[2] + k)));
-        c128 d = cvt<double>(dd[r]);
+        c128 d;
+        if constexpr (PRE_DD) d = cvt<double>(dd[r]);
+        else d = cvt<double>(ld_stream<NT>(dp + pbase + (size_t)j * N + k));
         if constexpr (MODE != BA_START) {
             const c128 tn = cvt<double>(v[r]) * J.inv_n;
             const c128 nd = update<PRE>(d, tn, ve, J.slow, J.eps, J.inv_eps);
@@ -374,27 +388,34 @@ __global__ void __launch_bounds__(PLANE_THREADS)
             const c128 p0 = pw_line * ld_g(J.pw[2] + k);
             v[r] = cvt<float>(source<PRE>(d, p0, ve, J.eps));
         }
+        if constexpr (MODE == BA_BOTH)
+            if ((r & 1) == 1) asm volatile("" ::: "memory");  // (b)
     }
     if constexpr (MODE != BA_END) {
         if constexpr (MODE != BA_START) __syncthreads();  // LDS free again
-        P::forward(v, t, lds + j * PLANE_LS, tw);
+        // (c) lane coordinates from an opaque copy of the thread index
+        int tid2 = tid;
+        if constexpr (MODE != BA_START) asm volatile("" : "+v"(tid2));
+        const int j2 = tid2 / P::TPL, t2 = tid2 % P::TPL;
+        const int kz2 = tid2 % N, ty2 = tid2 / N;
+        P::forward(v, t2, lds + j2 * PLANE_LS, tw);
         // transpose: z lines (outputs at in_pos<RZ>) -> y lines (inputs at in_pos<RA>)
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
-                lds[j * PLANE_TS + P::template in_pos<RZ>(t, u, r)] = v[u * RZ + r];
+                lds[j2 * PLANE_TS + P::template in_pos<RZ>(t2, u, r)] = v[u * RZ + r];
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < RA; ++r) v[r] = lds[P::template in_pos<RA>(ty, 0, r) * PLANE_TS + kz];
+        for (int r = 0; r < RA; ++r) v[r] = lds[P::template in_pos<RA>(ty2, 0, r) * PLANE_TS + kz2];
         __syncthreads();
-        P::forward(v, ty, lds + kz * PLANE_LS, tw);
+        P::forward(v, ty2, lds + kz2 * PLANE_LS, tw);
 #pragma unroll
         for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
-                store_nt(&tmp[pbase + (size_t)P::template in_pos<RZ>(ty, u, r) * N + kz],
+                store_nt(&tmp[pbase + (size_t)P::template in_pos<RZ>(ty2, u, r) * N + kz2],
                          v[u * RZ + r]);
     }
     if constexpr (MODE != BA_START) {
@@ -685,10 +706,11 @@ int plane(int mode, int pre, const BornDev *tab, int nact, const c64 *Vf, const 
     if (geo.dim != 3 || geo.n1 != (size_t)PLANE_N || geo.n2 != (size_t)PLANE_N)
         return set_error(QSIM_ERR_UNSUPPORTED, "plane sweep needs 128 x 128 y-z planes");
     if (mode == BA_START) return launch_plane_m<BA_START>(pre, tab, nact, Vf, geo, tw, s);
-    // the merged end + start needs more than the 128 registers a 1024-thread
-    // workgroup allows (it spilled ~200 B per lane), so it runs as two
-    // launches: 56 B per point instead of 40, still below the 80 of the
-    // inner + middle passes it replaces
+    // end + start in one pass (40 B per point instead of 56); Hao2's merged
+    // pass spills ~550 B per lane at the 128-register limit of a
+    // 1024-thread workgroup, so Hao2 keeps two launches
+    if (mode == BA_BOTH && pre != PRE_HAO2)
+        return launch_plane_m<BA_BOTH>(pre, tab, nact, Vf, geo, tw, s);
     QSIM_TRY(launch_plane_m<BA_END>(pre, tab, nact, Vf, geo, tw, s));
     if (mode == BA_END) return QSIM_OK;
     return launch_plane_m<BA_START>(pre, tab, nact, Vf, geo, tw, s);
