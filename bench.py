@@ -707,6 +707,8 @@ def main():
     ap.add_argument("--probe-n", type=int, default=1024, choices=(1024, 2048))
     ap.add_argument("--fft-probe", action="store_true",
                     help="run only FourierTransform plans of 1024^2, 2048^2, 128^3 (rocprofv3)")
+    ap.add_argument("--born-probe", action="store_true",
+                    help="run only the C4 Born section (for rocprofv3) and exit")
     ap.add_argument("--serving-probe", type=int, metavar="NREQ",
                     help="run only the request loop over NREQ requests (for rocprofv3) and exit")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -723,6 +725,12 @@ def main():
         return
     if args.fft_probe:
         fft_probe(max(args.probe_steps, 1))
+        return
+    if args.born_probe:
+        from qusim_amd import _native as Nn
+        Nn.init(0)
+        b = born_section()
+        print(json.dumps(b), flush=True)
         return
     if args.serving_probe:
         from qusim_amd import _native as Nn
