@@ -217,3 +217,36 @@ def test_delta_norm_2d_on_device_matches_oracle(oracle):
                                   preconditional=1, preconditioner=2, slow=0.7, epsilon=0.01)
         assert s.GetDeltaPsiNorm() == pytest.approx(dn, rel=1e-10), calls
     assert math.isfinite(s.GetDeltaPsiNorm())
+
+
+# ------------------------------------------------------------ imaginary time
+def test_imaginary_time_ground_state_on_device(oracle):
+    """Complex dt = -0.05 i with FNES after every call of 10 steps: the fused
+    1D kernel (256 points) and the fused 2D sweeps (128 x 96) relax a moving,
+    displaced packet to the oscillator ground state; psi matches the oracle
+    run with the same calls to 1e-12, and the device KinEn + PotEn is the
+    ground-state energy w / 2 ((wx + wy) / 2) to 1e-6."""
+    n, L, w = 256, 20.0, 1.3
+    x = -L / 2 + L / n * np.arange(n)
+    V = 0.5 * w * w * x * x
+    psi = np.exp(-(x - 1.5) ** 2 / 2 + 0.7j * x)
+    e = evolver1d(psi, V, -0.05j, batch=10, fnes=True, x0=-L / 2, x1=L / 2)
+    for _ in range(30):
+        e.step()
+    p0 = psi / np.sqrt(np.sum(np.abs(psi) ** 2) * L / n)
+    ref = oracle.split1d(p0, V, -L / 2, L / 2, -0.05j, 30, method=2, batch=10, fnes=True)
+    assert rel_l2(e.GetPsi(), ref) < 1e-12
+    assert e.KinEn() + e.PotEn() == pytest.approx(w / 2, abs=1e-6)
+    nx, ny, Lx, Ly, wx, wy = 128, 96, 16.0, 14.0, 1.0, 1.6
+    gx = -Lx / 2 + Lx / nx * np.arange(nx)
+    gy = -Ly / 2 + Ly / ny * np.arange(ny)
+    X, Y = np.meshgrid(gx, gy, indexing="ij")
+    V2 = 0.5 * (wx * wx * X * X + wy * wy * Y * Y)
+    p2 = np.exp(-((X - 1) ** 2 + (Y + 0.5) ** 2) / 3 + 0.4j * X)
+    e2 = evolver2d(p2, V2, -0.05j, batch=10, fnes=True, ext=(-Lx / 2, Lx / 2, -Ly / 2, Ly / 2))
+    e2.steps(30)
+    q0 = p2 / np.sqrt(np.sum(np.abs(p2) ** 2) * (Lx / nx) * (Ly / ny))
+    ref2 = oracle.split2d(q0, V2, -Lx / 2, Lx / 2, -Ly / 2, Ly / 2, -0.05j, 30, method=2,
+                          batch=10, fnes=True)
+    assert rel_l2(e2.GetPsi(), ref2) < 1e-12
+    assert e2.KinEn() + e2.PotEn() == pytest.approx((wx + wy) / 2, abs=1e-6)

@@ -471,3 +471,40 @@ def test_total_xsection_closed_forms(oracle):
     assert got2 == pytest.approx(disc2, rel=1e-9)
     # int_0^{2 pi} exp(-p^2 (1 - cos t) / a) dt = 2 pi e^{-p^2/a} I0(p^2/a)
     assert got2 == pytest.approx(c2 * 2 * math.pi * i0e(p * p / a), rel=1e-9)
+
+
+def test_imaginary_time_reaches_the_oscillator_ground_state(oracle):
+    """Complex dt = -i tau with FNES (src/EvolverImpl.cpp:56-68) is
+    imaginary-time propagation: a displaced, moving packet in
+    V = (w x)^2 / 2 (m = hbar = 1) relaxes to the ground state
+    exp(-w x^2 / 2) with energy w / 2 (2D: (wx + wy) / 2).  Strang splitting
+    in imaginary time has no negative sub-steps; the fourth-order scheme's
+    negative ones (D2 < 0, src/SplittingUtils.cpp:3-13) amplify high
+    momenta, so only O2 is a ground-state solver."""
+    n, L, w = 256, 20.0, 1.3
+    x = -L / 2 + L / n * np.arange(n)
+    dx = L / n
+    V = 0.5 * w * w * x * x
+    psi = np.exp(-(x - 1.5) ** 2 / 2 + 0.7j * x)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * dx)
+    g = np.exp(-w * x * x / 2)
+    g /= np.sqrt(np.sum(g * g) * dx)
+    out = oracle.split1d(psi, V, -L / 2, L / 2, -0.05j, 300, method=2, fnes=True)
+    kin, pot, nrm, xavg = oracle.observables(out, V, -L / 2, L / 2)
+    assert nrm == pytest.approx(1.0, abs=1e-13)
+    assert kin + pot == pytest.approx(w / 2, abs=1e-6)
+    # virial theorem <T> = <V>; the splitting's fixed point shifts each by O(tau^2)
+    assert kin == pytest.approx(w / 4, abs=1e-3) and pot == pytest.approx(w / 4, abs=1e-3)
+    assert abs(np.sum(np.conj(g) * out) * dx) ** 2 > 1 - 1e-6
+    assert abs(xavg) < 1e-6
+    nx, ny, Lx, Ly, wx, wy = 128, 96, 16.0, 14.0, 1.0, 1.6
+    gx = -Lx / 2 + Lx / nx * np.arange(nx)
+    gy = -Ly / 2 + Ly / ny * np.arange(ny)
+    X, Y = np.meshgrid(gx, gy, indexing="ij")
+    V2 = 0.5 * (wx * wx * X * X + wy * wy * Y * Y)
+    p2 = np.exp(-((X - 1) ** 2 + (Y + 0.5) ** 2) / 3 + 0.4j * X)
+    p2 /= np.sqrt(np.sum(np.abs(p2) ** 2) * (Lx / nx) * (Ly / ny))
+    out2 = oracle.split2d(p2, V2, -Lx / 2, Lx / 2, -Ly / 2, Ly / 2, -0.05j, 300, method=2,
+                          fnes=True)
+    kin2, pot2, _, _ = oracle.observables(out2, V2, -Lx / 2, Lx / 2, -Ly / 2, Ly / 2)
+    assert kin2 + pot2 == pytest.approx((wx + wy) / 2, abs=1e-6)
