@@ -86,6 +86,7 @@ def ref():
         R.ref_split2d_cv_run.argtypes = [_SZ, _SZ, _D, _D, _D, _D, _D, _D, _D, _D, _I, _VP,
                                          _VP, _SZ, _SZ, _I]
         R.ref_observables.argtypes = [_I, _SZ, _SZ, _D, _D, _D, _D, _D, _D, _I, _VP, _VP, _VP]
+        R.ref_device_ops.argtypes = [_SZ, _SZ, _SZ, _VP, _VP, _VP, _VP, _VP]
         _ref = R
     return _ref
 
@@ -323,6 +324,24 @@ def ref_born(V, lo, hi, energy, direction, order, ncalls=1, preconditional=True,
                             ncalls)
     assert rc == 0
     return dp, veff, eps.value, dn.value
+
+
+# Slots of the device-op battery (oracle/device_op_battery.h)
+OPS_ARRAYS = 41
+OPS_SCALARS = 18
+
+
+def ref_device_ops(dims, a, b, c, r):
+    """Every Device op of the compiled reference's CpuSeq device on the
+    battery's inputs: (arrays [OPS_ARRAYS, n], scalars [OPS_SCALARS])."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    a, b, c = _c(a), _c(b), _c(c)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(n * OPS_ARRAYS + OPS_SCALARS, np.complex128)
+    rc = ref().ref_device_ops(nx, ny, nz, _p(a), _p(b), _p(c), _p(r), _p(out))
+    assert rc == 0
+    return out[:n * OPS_ARRAYS].reshape(OPS_ARRAYS, n), out[n * OPS_ARRAYS:]
 
 
 def ref_xsection3d(psi0, dpsi, V, lo, hi, energy, cosx, cosy, cosz, mass=1.0, hbar=1.0):

@@ -27,6 +27,8 @@
 #include "SplittingUtils.h"
 #include "Utils.h"
 
+#include "device_op_battery.h"
+
 using Dev = DeviceTemplate<std::execution::sequenced_policy>;
 using Cx = std::complex<double>;
 
@@ -223,6 +225,16 @@ struct Split2D {
 } // namespace
 
 extern "C" {
+
+// Every Device op of the CpuSeq device (src/DeviceTemplate.h) on the battery's
+// fixed inputs (oracle/device_op_battery.h).
+int ref_device_ops(size_t nx, size_t ny, size_t nz, const Cx* a, const Cx* b, const Cx* c,
+                   const double* r, Cx* out)
+{
+    Dev dev;
+    device_op_battery(dev, nx, ny, nz, a, b, c, r, out);
+    return 0;
+}
 
 int ref_fft(int rank, const size_t* dims, int inverse, const Cx* in, Cx* out)
 {

@@ -10,6 +10,9 @@
 //   born3d <n> <L> <E> <eps> <slow> <order> <in.bin> <out.bin>
 //     PreconditionalBornSeries::Update3D with Vellekoop (src/Perburbation.h:350-531)
 //     from precomputed psi0 and V_eff; input: psi0 then V_eff (n^3 c128 each).
+//   ops <nx> <ny> <nz> <in.bin> <out.bin>
+//     every Device op once on fixed inputs (oracle/device_op_battery.h);
+//     input: a, b, c (n c128 each) then r (n f64).
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -19,6 +22,7 @@
 #include <vector>
 
 #include "hip_device.h"
+#include "../../oracle/device_op_battery.h"
 
 using namespace qsim_host;
 using cd = std::complex<double>;
@@ -112,9 +116,25 @@ static int born3d(char **a)
     return 0;
 }
 
+static int ops(char **a)
+{
+    size_t nx = std::atol(a[0]), ny = std::atol(a[1]), nz = std::atol(a[2]);
+    size_t N = nx * ny * nz;
+    auto in = slurp(a[3]);
+    if (in.size() != N * (3 * sizeof(cd) + sizeof(double))) throw std::runtime_error("bad input size");
+    const cd *pa = (const cd *)in.data();
+    const double *pr = (const double *)(in.data() + 3 * N * sizeof(cd));
+    std::vector<cd> out(N * kBatteryArrays + kBatteryScalars);
+    HipDevice dev;
+    device_op_battery(dev, nx, ny, nz, pa, pa + N, pa + 2 * N, pr, out.data());
+    spit(a[4], out.data(), out.size() * sizeof(cd));
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     try {
+        if (argc >= 7 && std::strcmp(argv[1], "ops") == 0) return ops(argv + 2);
         if (argc >= 12 && std::strcmp(argv[1], "split2d") == 0) return split2d(argv + 2);
         if (argc >= 10 && std::strcmp(argv[1], "born3d") == 0) return born3d(argv + 2);
         std::fprintf(stderr, "usage: split2d ... | born3d ...\n");

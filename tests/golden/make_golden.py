@@ -125,6 +125,17 @@ def main():
             psi0 = O.plane_wave(Vb3.shape, lo3, hi3, 4.0, (0.0, 0.2, 1.0))
             out["b3_vell_xsec"] = np.array([O.ref_xsection3d(psi0, dp, veff, lo3, hi3, 4.0,
                                                              0.3, 0.1, 0.9)])
+    # Every Device op of the CpuSeq device on fixed inputs (n = 3 x 4 x 5),
+    # the battery of oracle/device_op_battery.h
+    dims = (3, 4, 5)
+    m = np.arange(60, dtype=np.float64)
+    oa = np.sin(0.37 * m + 0.1) + 1j * np.cos(0.91 * m)
+    ob = np.cos(0.23 * m - 0.4) + 0.5j * np.sin(0.71 * m + 0.2)
+    oc = 0.3 * np.exp(-((m - 30) / 12) ** 2) + 0.05j * np.cos(0.17 * m)
+    orr = 0.5 + 0.4 * np.sin(0.53 * m) ** 2
+    arrays, scalars = O.ref_device_ops(dims, oa, ob, oc, orr)
+    out.update(ops_dims=np.array(dims), ops_a=oa, ops_b=ob, ops_c=oc, ops_r=orr,
+               ops_arrays=arrays, ops_scalars=scalars)
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes,", len(out), "arrays")

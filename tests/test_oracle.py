@@ -277,6 +277,11 @@ def test_compiled_reference_agrees_when_present(oracle):
     V = rng.uniform(0, 1, n) * 0.3
     assert rel_l2(oracle.split1d(psi, V, -50.0, 50.0, 0.07, 120, method=4),
                   oracle.ref_split1d(psi, V, -50.0, 50.0, 0.07, 120, method=4)) < 1e-12
+    # the committed Device-op battery is what the compiled reference gives
+    g = np.load(GOLD)
+    arrays, scalars = oracle.ref_device_ops(tuple(int(d) for d in g["ops_dims"]), g["ops_a"],
+                                            g["ops_b"], g["ops_c"], g["ops_r"])
+    assert np.array_equal(arrays, g["ops_arrays"]) and np.array_equal(scalars, g["ops_scalars"])
 
 
 # ------------------------------------------------ complex potential (C2 / C3)
@@ -508,3 +513,26 @@ def test_imaginary_time_reaches_the_oscillator_ground_state(oracle):
                           fnes=True)
     kin2, pot2, _, _ = oracle.observables(out2, V2, -Lx / 2, Lx / 2, -Ly / 2, Ly / 2)
     assert kin2 + pot2 == pytest.approx((wx + wy) / 2, abs=1e-6)
+
+
+def test_device_op_battery_golden_slots():
+    """The compiled reference's Device-op battery (oracle/device_op_battery.h)
+    holds what its slots name: spot checks of ops whose result is a plain
+    formula, so a shifted slot or input cannot pass for the GPU comparison."""
+    g = np.load(GOLD)
+    a, b, r = g["ops_a"], g["ops_b"], g["ops_r"]
+    A, S = g["ops_arrays"], g["ops_scalars"]
+    assert np.array_equal(A[0], np.zeros_like(a)) and np.array_equal(A[1], np.ones_like(a))
+    assert np.allclose(A[2], np.exp((0.3 - 0.7j) * a), rtol=1e-15, atol=0)
+    assert np.array_equal(A[5], a)
+    assert np.array_equal(A[10], a - b)
+    assert np.allclose(A[11], a * b, rtol=1e-15, atol=1e-16)
+    assert np.array_equal(A[21], -a[::-1])
+    assert np.array_equal(A[36], r.astype(np.complex128))
+    expect_set = a.copy()
+    expect_set[3] = 5 - 6j
+    assert np.array_equal(A[39], expect_set)
+    assert S[2] == a[4] and S[3] == r[5]
+    assert S[6] == pytest.approx(np.sum(a.real), rel=1e-14)
+    assert S[10] == pytest.approx(np.sum(np.abs(a) ** 2), rel=1e-14)
+    assert S[8] == pytest.approx(np.sum(np.abs(a - b) ** 2), rel=1e-14)
