@@ -67,10 +67,14 @@ __device__ __forceinline__ c128 psi0_at(const BornDev &J, int dim, size_t i, siz
                     : ld_g(J.pw[0] + i) * ld_g(J.pw[1] + j);
 }
 
-// gamma of op_Vellekoop / op_Hao1 (src/DeviceTemplate.h:1020-1052) with the
-// divisions by eps done as products with 1/eps: within an ulp of the
-// quotient, without two FP64 divide sequences per point in the fused sweeps
-// (op_Hao2 keeps its complex division)
+// gamma of op_Vellekoop / op_Hao1 / op_Hao2 (src/DeviceTemplate.h:1020-1077)
+// with the divisions by eps done as products with 1/eps: within an ulp of the
+// quotient, without two FP64 divide sequences per point in the fused sweeps.
+// op_Hao2's 2 / w, w = 1 + i v / eps = (1 - v.y / eps, v.x / eps), is taken
+// as 2 conj(w) / |w|^2 with one real divide instead of a complex division
+// (|w|^2 >= 1 for the absorbing potentials, Im v <= 0, so the unscaled form
+// neither overflows nor underflows); that also frees the registers the
+// merged plane pass needs.
 template <int PRE>
 __device__ __forceinline__ c128 gamma_fused(c128 v, double slow, double eps, double inv_eps)
 {
@@ -81,7 +85,9 @@ __device__ __forceinline__ c128 gamma_fused(c128 v, double slow, double eps, dou
         const c128 iv = imul(mk(v.x, -v.y));
         return mk(1. - iv.x * inv_eps, -iv.y * inv_eps) * slow;
     } else {
-        return born_gamma<PRE>(v, slow, eps);
+        const double wr = 1. - v.y * inv_eps, wi = v.x * inv_eps;
+        const double s2 = 2. * slow / (wr * wr + wi * wi);
+        return mk(wr * s2, -wi * s2);
     }
 }
 
@@ -283,7 +289,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
 // line-fast (lane -> z position), so every load and store is coalesced; the
 // switch between y lines and z lines is a transpose through LDS.
 constexpr int PLANE_N = 128;
-using PlaneLine = Pow2Line<float, PLANE_N, 16, 8, 1>;   // E = 16, 8 threads per line
+using PlaneLine = Pow2Line<float, PLANE_N, 16, 8, 1, 1, true>;  // E = 16, 8 threads per line
 constexpr int PLANE_THREADS = PLANE_N * PlaneLine::TPL;   // 1024
 constexpr int PLANE_LS = PlaneLine::LS;                   // exchange stride (1 mod 16)
 constexpr int PLANE_TS = PLANE_N + 1;                     // transpose row stride
@@ -292,25 +298,26 @@ constexpr size_t PLANE_LDS =
     sizeof(c64);
 
 // The plane kernels sit at the 128-register limit of a 1024-thread
-// workgroup.  With the recurrence form of the twiddle powers the END pass
-// spilled 28 B per lane (Vellekoop, Hao1); the product-tree form leaves it
-// without spills (C4: +2.5%).  Hao2's END spills more with the tree (76 B
-// against 12 B), so it keeps the recurrence.  BOTH fits the same budget
-// (Vellekoop, Hao1: 56 B per lane of scratch, against 220-350 B) because
-// (a) it reads each point's dpsi in the update loop instead of prefetching
-// the line, (b) compiler barriers every two points keep the update's loads
-// from being hoisted all at once, and (c) START takes its lane coordinates
-// from an opaque copy of the thread index, so END's address offsets are
-// recomputed rather than held across the pass.
-template <int PRE>
-using PlaneLineFor = Pow2Line<float, PLANE_N, 16, 8, 1, 1, PRE != PRE_HAO2>;
+// workgroup.  They take the product-tree form of the twiddle powers: with
+// the recurrence form END spilled 28 B per lane (Vellekoop, Hao1), the tree
+// leaves it without spills (C4: +2.5%).  The recurrence form does leave
+// BOTH at 82 registers with no spill, but C4 ran at 2.95e4
+// energy-iterations/s with it against 3.58e4 with the tree's 28 B of spill.  BOTH fits the budget (Vellekoop, Hao1:
+// 28 B per lane of scratch, against 220-350 B; Hao2 none) because (a) it
+// reads each point's dpsi in the update loop instead of prefetching the
+// line, (b) the inverse transform's outputs pass through an opaque register
+// barrier before the update loop, so its last butterflies are not
+// interleaved with the update's loads, and compiler barriers every two
+// points keep those loads from being hoisted all at once, and (c) START
+// takes its lane coordinates from an opaque copy of the thread index, so
+// END's address offsets are recomputed rather than held across the pass.
 
 template <int MODE, int PRE, bool NT>
 __global__ void __launch_bounds__(PLANE_THREADS)
     k_born_plane(const BornDev *__restrict__ jobs, const c64 *__restrict__ Vf, BornGeom geo,
                  const c128 *__restrict__ tw)
 {
-    using P = PlaneLineFor<PRE>;
+    using P = PlaneLine;
     constexpr int N = PLANE_N;
     constexpr int RA = P::RA_, RZ = P::RLAST;
     extern __shared__ unsigned char smem_raw[];
@@ -362,7 +369,10 @@ __global__ void __launch_bounds__(PLANE_THREADS)
         __syncthreads();
         P::inverse(v, t, lds + j * PLANE_LS, tw);
     }
-    if constexpr (MODE == BA_BOTH) asm volatile("" ::: "memory");  // (b)
+    if constexpr (MODE == BA_BOTH) {  // (b)
+#pragma unroll
+        for (int r = 0; r < P::E; ++r) asm volatile("" : "+v"(v[r].x), "+v"(v[r].y) :: "memory");
+    }
     const double ab_line = ld_g(J.ab[0] + i) + ld_g(J.ab[1] + j);
     const c128 pw_line = ld_g(J.pw[0] + i) * ld_g(J.pw[1] + j);
     double rsum = 0, nsum = 0;
@@ -706,11 +716,8 @@ int plane(int mode, int pre, const BornDev *tab, int nact, const c64 *Vf, const 
     if (geo.dim != 3 || geo.n1 != (size_t)PLANE_N || geo.n2 != (size_t)PLANE_N)
         return set_error(QSIM_ERR_UNSUPPORTED, "plane sweep needs 128 x 128 y-z planes");
     if (mode == BA_START) return launch_plane_m<BA_START>(pre, tab, nact, Vf, geo, tw, s);
-    // end + start in one pass (40 B per point instead of 56); Hao2's merged
-    // pass spills ~550 B per lane at the 128-register limit of a
-    // 1024-thread workgroup, so Hao2 keeps two launches
-    if (mode == BA_BOTH && pre != PRE_HAO2)
-        return launch_plane_m<BA_BOTH>(pre, tab, nact, Vf, geo, tw, s);
+    // end + start in one pass (40 B per point instead of 56)
+    if (mode == BA_BOTH) return launch_plane_m<BA_BOTH>(pre, tab, nact, Vf, geo, tw, s);
     QSIM_TRY(launch_plane_m<BA_END>(pre, tab, nact, Vf, geo, tw, s));
     if (mode == BA_END) return QSIM_OK;
     return launch_plane_m<BA_START>(pre, tab, nact, Vf, geo, tw, s);

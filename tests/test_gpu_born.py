@@ -133,23 +133,25 @@ def test_fp32_iterates_error(oracle):
     assert err < 1e-5
 
 
-def test_fp32_plane_sweep_matches_oracle(oracle):
+@pytest.mark.parametrize("pre,slow", [(1, 0.5), (2, 0.7), (3, 0.5)])
+def test_fp32_plane_sweep_matches_oracle(pre, slow, oracle):
     """3D FP32 jobs with 128 x 128 y-z planes run the plane sweep (both
-    transforms of a plane in one workgroup, END and START launches); several
-    energies per launch, each against its own FP64 oracle run"""
+    transforms of a plane in one workgroup: a START launch, merged END+START
+    launches, an END launch) for each preconditioner; several energies per
+    launch, each against its own FP64 oracle run"""
     from qusim_amd.api import QuPerturbation3D, SolverMethod
     shape, lo, hi = (64, 128, 128), (-16.0, -32.0, -32.0), (16.0, 32.0, 32.0)
     V = pot3d(shape, lo, hi)
     Es = [1.5, 3.0]
     s = QuPerturbation3D().init(V, lo[0], hi[0], shape[0], lo[1], hi[1], shape[1], lo[2], hi[2],
                                 shape[2], Es, 0.0, 0, 0, 1, SolverMethod.BornSerise, 1.0, 1.0,
-                                opts(1, 1, 0.5, 12, precision="mixed32"))
+                                opts(1, pre, slow, 12, precision="mixed32"))
     s.Compute()
     for k, E in enumerate(Es):
         ref, veff, _, _ = oracle.born(V, lo, hi, E, (0, 0, 1), order=12, preconditional=1,
-                                      preconditioner=1, slow=0.5, threads=8)
+                                      preconditioner=pre, slow=slow, threads=8)
         err = rel_l2(s.GetPsi(k), ref)
-        print("fp32 plane sweep (64,128,128) E=%g x12 rel L2 %.2e" % (E, err))
+        print("fp32 plane sweep (64,128,128) pre=%d E=%g x12 rel L2 %.2e" % (pre, E, err))
         assert err < 1e-5
         psi0 = oracle.plane_wave(shape, lo, hi, E, (0, 0, 1))
         xs, _ = oracle.xsection3d(psi0, ref, veff, lo, hi, E, 0.0, 0.0, 1.0)
