@@ -106,10 +106,16 @@ struct P2Launch {
     static constexpr int L = cmin(L_CAP, L_FIXED > 0 ? L_FIXED : pow2floor(cmin(L_WANT, L_LDS)));
     static constexpr int THREADS = L * Line::TPL;
     // LDS stride between the lines of a workgroup (in elements).  Line-fast
-    // tiles put lane i on line i % L, element i / L: a stride of 16/L (mod 16)
-    // sends the lanes of one 16-lane group to 16 different 16-byte slots;
-    // rows keep 1 (mod 16).
-    static constexpr int TARGET = LF ? (L >= 16 ? 1 : 16 / L) : 1;
+    // tiles put lane i on line i % L, element i / L.  The stride (mod 16)
+    // is the one that took the fewest LDS cycles over every exchange of
+    // these tiles, measured per element size with rocprofv3
+    // (SQ_LDS_IDX_ACTIVE per instruction, stores and loads timed apart):
+    // 8-byte elements 16/L; 16-byte elements 8/L for L = 2 and 4 and 7 for
+    // L = 8 (FP64 512-point tiles: 2.0 -> 1.33 times the conflict-free
+    // cycles, 4-line tiles 1.67 -> 1.33).  Rows keep 1 (mod 16).
+    static constexpr int TARGET =
+        !LF ? 1
+            : (sizeof(cx<T>) == 8 ? (L >= 16 ? 1 : 16 / L) : (L >= 8 ? (L == 8 ? 7 : 1) : 8 / L));
     static constexpr int LSTR =
         Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
     static constexpr size_t LDS_BYTES = (size_t)L * LSTR * sizeof(cx<T>);

@@ -460,8 +460,10 @@ __device__ __forceinline__ void row_group_xl(const SplitDev &J, size_t i2, size_
 }
 
 // Launch shape of the x-split row sweep: R lines of Line P per workgroup,
-// lanes line-fast within each quad; the LDS line stride sends the 16 lanes
-// of a group to 16 distinct 16-byte slots (as the line-fast column tiles).
+// lanes line-fast within each quad.  The LDS line stride (mod 16) is the
+// measured best of the exchanges, as for the line-fast column tiles: 16/R
+// for 8-byte elements, 8/R for 16-byte ones (FP64 1024-point rows, R = 2:
+// 2.17 -> 1.50 times the conflict-free LDS cycles).
 template <typename T, int N, int V, int R>
 struct XRowLaunch {
     using Line = typename P2Cfg<T, N, V>::Line;
@@ -469,7 +471,7 @@ struct XRowLaunch {
     // 8192-point rows spilled more in that layout than one lane per row)
     static constexpr bool LOCAL = N == 4096;
     static constexpr int THREADS = LOCAL ? Line::TPL : R * Line::TPL;
-    static constexpr int TARGET = 16 / R;
+    static constexpr int TARGET = (sizeof(cx<T>) == 8 ? 16 : 8) / R;
     static constexpr int LSTR =
         LOCAL ? Line::LS : Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
     static constexpr size_t LDS_BYTES = (size_t)(LOCAL ? 1 : R) * LSTR * sizeof(cx<T>);
