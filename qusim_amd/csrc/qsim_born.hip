@@ -291,11 +291,21 @@ __global__ void __launch_bounds__((P2Launch<T, N, false, BornV<N>::V>::THREADS))
 constexpr int PLANE_N = 128;
 using PlaneLine = Pow2Line<float, PLANE_N, 16, 8, 1, 1, true>;  // E = 16, 8 threads per line
 constexpr int PLANE_THREADS = PLANE_N * PlaneLine::TPL;   // 1024
-constexpr int PLANE_LS = PlaneLine::LS;                   // exchange stride (1 mod 16)
-constexpr int PLANE_TS = PLANE_N + 1;                     // transpose row stride
-constexpr size_t PLANE_LDS =
-    (size_t)(PLANE_N * PLANE_LS > PLANE_N * PLANE_TS ? PLANE_N * PLANE_LS : PLANE_N * PLANE_TS) *
-    sizeof(c64);
+// LDS strides, each the one that took the fewest LDS cycles for its
+// accesses (every store and load pattern timed under each stride mod 16 with
+// rocprofv3 SQ_LDS_IDX_ACTIVE): z-line exchanges (8 lines per wave, lane =
+// 8 j + t) 8 mod 16, 2.0 -> 1.0 times the conflict-free cycles against
+// 1 mod 16; y-line exchanges (64 lines per wave, lane = line) odd; the
+// transpose rows 8 mod 16, 1.83 -> 1.0 against PLANE_N + 1.
+constexpr int PLANE_LSY = PlaneLine::LS;                  // y-line stride (1 mod 16)
+constexpr int PLANE_LSZ =
+    PlaneLine::LS_RAW + ((8 - PlaneLine::LS_RAW % 16) % 16 + 16) % 16;  // z lines (8 mod 16)
+constexpr int PLANE_TS = PLANE_N + 8;                     // transpose row stride
+constexpr int PLANE_ELEMS = cmax(PLANE_N * cmax(PLANE_LSY, PLANE_LSZ), PLANE_N * PLANE_TS);
+constexpr size_t PLANE_LDS = (size_t)PLANE_ELEMS * sizeof(c64);
+static_assert(PLANE_LSY % 16 == 1 && PLANE_LSZ % 16 == 8, "plane LDS strides");
+static_assert(PLANE_LDS + 2 * (PLANE_THREADS / 64) * sizeof(double) <= 160 * 1024,
+              "plane sweep LDS within one CU");
 
 // The plane kernels sit at the 128-register limit of a 1024-thread
 // workgroup.  They take the product-tree form of the twiddle powers: with
@@ -355,7 +365,7 @@ __global__ void __launch_bounds__(PLANE_THREADS)
 #pragma unroll
             for (int r = 0; r < RZ; ++r)
                 v[u * RZ + r] = ld_stream<NT>(tmp + pbase + (size_t)P::template in_pos<RZ>(ty, u, r) * N + kz);
-        P::inverse(v, ty, lds + kz * PLANE_LS, tw);
+        P::inverse(v, ty, lds + kz * PLANE_LSY, tw);
         // transpose: y lines (outputs at in_pos<RA>) -> z lines (inputs at in_pos<RZ>)
         __syncthreads();
 #pragma unroll
@@ -367,7 +377,7 @@ __global__ void __launch_bounds__(PLANE_THREADS)
             for (int r = 0; r < RZ; ++r)
                 v[u * RZ + r] = lds[j * PLANE_TS + P::template in_pos<RZ>(t, u, r)];
         __syncthreads();
-        P::inverse(v, t, lds + j * PLANE_LS, tw);
+        P::inverse(v, t, lds + j * PLANE_LSZ, tw);
     }
     if constexpr (MODE == BA_BOTH) {  // (b)
 #pragma unroll
@@ -408,7 +418,7 @@ __global__ void __launch_bounds__(PLANE_THREADS)
         if constexpr (MODE != BA_START) asm volatile("" : "+v"(tid2));
         const int j2 = tid2 / P::TPL, t2 = tid2 % P::TPL;
         const int kz2 = tid2 % N, ty2 = tid2 / N;
-        P::forward(v, t2, lds + j2 * PLANE_LS, tw);
+        P::forward(v, t2, lds + j2 * PLANE_LSZ, tw);
         // transpose: z lines (outputs at in_pos<RZ>) -> y lines (inputs at in_pos<RA>)
         __syncthreads();
 #pragma unroll
@@ -420,7 +430,7 @@ __global__ void __launch_bounds__(PLANE_THREADS)
 #pragma unroll
         for (int r = 0; r < RA; ++r) v[r] = lds[P::template in_pos<RA>(ty2, 0, r) * PLANE_TS + kz2];
         __syncthreads();
-        P::forward(v, ty2, lds + kz2 * PLANE_LS, tw);
+        P::forward(v, ty2, lds + kz2 * PLANE_LSY, tw);
 #pragma unroll
         for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
