@@ -138,23 +138,40 @@ __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 // w^(r/2) w^(r - r/2), all R-1 powers live) instead of the recurrence
 // w^r = w^(r-1) w (one live power).  Same multiplies, other register
 // allocation; a kernel opts in where that avoids spills.
-template <typename T, int N, int RA, int RB, int RC, int RD = 1, bool TREE = false>
+// SWZ: LDS placement of the exchanges.  0 pads one slot per R elements
+// after a radix-R pass; 1, 2 and 3 XOR-swizzle the low bits of the element
+// index inside its aligned 8- or 16-element block instead (no padding):
+// m ^ ((m >> 3) & 7), m ^ ((m >> 2) & 7), m ^ ((m >> 3) & 15).  Padding
+// keeps the stores free of conflicts but breaks the runs of consecutive
+// addresses that the loads want; a swizzle can keep both.
+template <typename T, int N, int RA, int RB, int RC, int RD = 1, bool TREE = false, int SWZ = 0>
 struct Pow2Line {
     static_assert(RA * RB * RC * RD == N, "radices must multiply to N");
     static_assert(RD == 1 || RC > 1, "a fourth pass needs a third");
     static constexpr int NP = RD > 1 ? 4 : (RC > 1 ? 3 : (RB > 1 ? 2 : 1));
     static constexpr int RA_ = RA;
+    // the same line with another LDS placement
+    template <int S>
+    using WithSwz = Pow2Line<T, N, RA, RB, RC, RD, TREE, S>;
     static constexpr int E = cmax(cmax(RA, RB), cmax(RC, RD));
     static constexpr int TPL = N / E;
     static constexpr int RMIN = cmin(cmin(RA, RB > 1 ? RB : RA),
                                      cmin(RC > 1 ? RC : RA, RD > 1 ? RD : RA));
-    // LDS elements per line: N plus one pad per RMIN, rounded to 1 mod 16 so
-    // lines placed side by side start on different banks.
-    static constexpr int LS_RAW = N + N / RMIN + 1;
+    // LDS elements per line: N (swizzled) or N plus one pad per RMIN, rounded
+    // to 1 mod 16 so lines placed side by side start on different banks.
+    static constexpr int LS_RAW = SWZ == 0 ? N + N / RMIN + 1 : N;
     static constexpr int LS = LS_RAW + ((1 - LS_RAW % 16) + 16) % 16;
 
+    // LDS index of line element m in the exchange after a radix-R pass;
+    // the swizzles stay inside m's aligned block, so indices stay below N
     template <int R>
-    static __device__ __forceinline__ int pad(int m) { return m + m / R; }
+    static __device__ __forceinline__ int pad(int m)
+    {
+        if constexpr (SWZ == 0) return m + m / R;
+        else if constexpr (SWZ == 1) return m ^ ((m >> 3) & 7);
+        else if constexpr (SWZ == 2) return m ^ ((m >> 2) & 7);
+        else return m ^ ((m >> 3) & 15);
+    }
 
     // Positions of the elements thread t holds as inputs of a first pass of
     // radix R: element (u, r) is at j + r*(N/R).

@@ -94,7 +94,13 @@ __host__ __device__ constexpr int pow2floor(int x) { return x <= 1 ? 1 : 2 * pow
 template <typename T, int N, bool LF, int V = 0>
 struct P2Launch {
     using Cfg = P2Cfg<T, N, V>;
-    using Line = typename Cfg::Line;
+    // rows with a wave or more per line take the XOR-swizzled exchanges
+    // (LDS cycles measured per store and load pattern: FP64 1024-point rows
+    // 1.33-1.61 -> 1.11-1.17 times the conflict-free count, FP32 1.56 ->
+    // 1.22); line-fast tiles and shorter rows keep the padding
+    static constexpr int SWZ =
+        (!LF && Cfg::Line::TPL >= 64) ? (sizeof(cx<T>) == 16 ? 1 : 3) : 0;
+    using Line = typename Cfg::Line::template WithSwz<SWZ>;
     static constexpr int LF_LINES = 128 / (int)sizeof(cx<T>);
     static constexpr int ROWS = cmax(1, 256 / Line::TPL);
     static constexpr int L_WANT = LF ? LF_LINES : ROWS;

@@ -466,10 +466,14 @@ __device__ __forceinline__ void row_group_xl(const SplitDev &J, size_t i2, size_
 // 2.17 -> 1.50 times the conflict-free LDS cycles).
 template <typename T, int N, int V, int R>
 struct XRowLaunch {
-    using Line = typename P2Cfg<T, N, V>::Line;
     // rows of 4096 points: the R rows in each thread (row_group_xl; FP32
     // 8192-point rows spilled more in that layout than one lane per row)
     static constexpr bool LOCAL = N == 4096;
+    // FP64 groups of rows, lane-interleaved, take the m ^ ((m >> 2) & 7)
+    // swizzle (1.50 -> 1.11 times the conflict-free LDS cycles at 1024
+    // points, R = 2); FP32 groups and the in-thread layout keep the padding
+    using Line = typename P2Cfg<T, N, V>::Line::template WithSwz<
+        (!LOCAL && sizeof(cx<T>) == 16) ? 2 : 0>;
     static constexpr int THREADS = LOCAL ? Line::TPL : R * Line::TPL;
     static constexpr int TARGET = (sizeof(cx<T>) == 8 ? 16 : 8) / R;
     static constexpr int LSTR =
