@@ -492,7 +492,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, BornLFV<N>::V>::THREADS)
         for (int r = 0; r < RA; ++r)
             v[u * RA + r] = active ? ld_stream<NT>(tmp + c + (size_t)P::template in_pos<RA>(t, u, r) * S)
                                    : mk<T>(0, 0);
-    P::forward(v, t, lds + l * LP::LSTR, tw);
+    P::forward(v, t, lds + l * LP::LSTR, tw, l);
 #pragma unroll
     for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
@@ -505,7 +505,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, BornLFV<N>::V>::THREADS)
             const c128 z = mul_g0(cvt<double>(v[u * RZ + r]), J.E - tt, J.eps);
             v[u * RZ + r] = cvt<T>(z);
         }
-    P::inverse(v, t, lds + l * LP::LSTR, tw);
+    P::inverse(v, t, lds + l * LP::LSTR, tw, l);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / RA; ++u)
@@ -544,9 +544,9 @@ __global__ void __launch_bounds__((P2Launch<T, N, true, BornLFV<N>::V>::THREADS)
             v[u * R0 + r] = active ? ld_g(tmp + base + (size_t)P::template in_pos<R0>(t, u, r) * S)
                                    : mk<T>(0, 0);
     if constexpr (INV)
-        P::inverse(v, t, lds + l * LP::LSTR, tw);
+        P::inverse(v, t, lds + l * LP::LSTR, tw, l);
     else
-        P::forward(v, t, lds + l * LP::LSTR, tw);
+        P::forward(v, t, lds + l * LP::LSTR, tw, l);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / R1; ++u)

@@ -141,9 +141,11 @@ __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 // SWZ: LDS placement of the exchanges.  0 pads one slot per R elements
 // after a radix-R pass; 1, 2 and 3 XOR-swizzle the low bits of the element
 // index inside its aligned 8- or 16-element block instead (no padding):
-// m ^ ((m >> 3) & 7), m ^ ((m >> 2) & 7), m ^ ((m >> 3) & 15).  Padding
-// keeps the stores free of conflicts but breaks the runs of consecutive
-// addresses that the loads want; a swizzle can keep both.
+// m ^ ((m >> 3) & 7), m ^ ((m >> 2) & 7), m ^ ((m >> 3) & 15); 4 XORs the
+// low 3 bits with a per-line key (0..7) that the caller passes (the line's
+// index in a line-fast tile).  Padding keeps the stores free of conflicts
+// but breaks the runs of consecutive addresses that the loads want; a
+// swizzle can keep both.
 template <typename T, int N, int RA, int RB, int RC, int RD = 1, bool TREE = false, int SWZ = 0>
 struct Pow2Line {
     static_assert(RA * RB * RC * RD == N, "radices must multiply to N");
@@ -160,17 +162,19 @@ struct Pow2Line {
     // LDS elements per line: N (swizzled) or N plus one pad per RMIN, rounded
     // to 1 mod 16 so lines placed side by side start on different banks.
     static constexpr int LS_RAW = SWZ == 0 ? N + N / RMIN + 1 : N;
+    static_assert(SWZ != 4 || N % 8 == 0, "per-line keys permute aligned 8-element blocks");
     static constexpr int LS = LS_RAW + ((1 - LS_RAW % 16) + 16) % 16;
 
     // LDS index of line element m in the exchange after a radix-R pass;
     // the swizzles stay inside m's aligned block, so indices stay below N
     template <int R>
-    static __device__ __forceinline__ int pad(int m)
+    static __device__ __forceinline__ int pad(int m, int key)
     {
         if constexpr (SWZ == 0) return m + m / R;
         else if constexpr (SWZ == 1) return m ^ ((m >> 3) & 7);
         else if constexpr (SWZ == 2) return m ^ ((m >> 2) & 7);
-        else return m ^ ((m >> 3) & 15);
+        else if constexpr (SWZ == 3) return m ^ ((m >> 3) & 15);
+        else return m ^ (key & 7);
     }
 
     // Positions of the elements thread t holds as inputs of a first pass of
@@ -237,7 +241,7 @@ struct Pow2Line {
     // LDS exchange from the outputs of a pass (radix R, after NS) to the
     // inputs of a pass of radix R2.
     template <int R, int NS, int R2>
-    static __device__ __forceinline__ void exchange(cx<T> *v, int t, cx<T> *lds)
+    static __device__ __forceinline__ void exchange(cx<T> *v, int t, cx<T> *lds, int key)
     {
         __syncthreads();
 #pragma unroll
@@ -245,31 +249,31 @@ struct Pow2Line {
             const int j = t + u * TPL;
             const int base = (j / NS) * NS * R + (j & (NS - 1));
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[pad<R>(base + r * NS)] = v[u * R + r];
+            for (int r = 0; r < R; ++r) lds[pad<R>(base + r * NS, key)] = v[u * R + r];
         }
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < E / R2; ++u)
 #pragma unroll
-            for (int r = 0; r < R2; ++r) v[u * R2 + r] = lds[pad<R>(in_pos<R2>(t, u, r))];
+            for (int r = 0; r < R2; ++r) v[u * R2 + r] = lds[pad<R>(in_pos<R2>(t, u, r), key)];
     }
 
     // Forward transform.  In: inputs of radix-RA pass (in_pos<RA>).  Out: X at
     // in_pos<RLAST>(t, u, r) with RLAST = RD, RC, RB or RA (NP = 4, 3, 2, 1).
     static __device__ __forceinline__ void forward(cx<T> *v, int t, cx<T> *lds,
-                                                   const c128 *__restrict__ tw)
+                                                   const c128 *__restrict__ tw, int key = 0)
     {
         pass<RA, 1, false>(v, t, tw);
         if constexpr (NP >= 2) {
-            exchange<RA, 1, RB>(v, t, lds);
+            exchange<RA, 1, RB>(v, t, lds, key);
             pass<RB, RA, false>(v, t, tw);
         }
         if constexpr (NP >= 3) {
-            exchange<RB, RA, RC>(v, t, lds);
+            exchange<RB, RA, RC>(v, t, lds, key);
             pass<RC, RA * RB, false>(v, t, tw);
         }
         if constexpr (NP == 4) {
-            exchange<RC, RA * RB, RD>(v, t, lds);
+            exchange<RC, RA * RB, RD>(v, t, lds, key);
             pass<RD, RA * RB * RC, false>(v, t, tw);
         }
     }
@@ -279,27 +283,27 @@ struct Pow2Line {
     // Inverse transform, radices RLAST, ..., RA.  In: in_pos<RLAST>.  Out: x at
     // in_pos<RA>(t, u, r).
     static __device__ __forceinline__ void inverse(cx<T> *v, int t, cx<T> *lds,
-                                                   const c128 *__restrict__ tw)
+                                                   const c128 *__restrict__ tw, int key = 0)
     {
         if constexpr (NP == 1) {
             pass<RA, 1, true>(v, t, tw);
         } else if constexpr (NP == 2) {
             pass<RB, 1, true>(v, t, tw);
-            exchange<RB, 1, RA>(v, t, lds);
+            exchange<RB, 1, RA>(v, t, lds, key);
             pass<RA, RB, true>(v, t, tw);
         } else if constexpr (NP == 3) {
             pass<RC, 1, true>(v, t, tw);
-            exchange<RC, 1, RB>(v, t, lds);
+            exchange<RC, 1, RB>(v, t, lds, key);
             pass<RB, RC, true>(v, t, tw);
-            exchange<RB, RC, RA>(v, t, lds);
+            exchange<RB, RC, RA>(v, t, lds, key);
             pass<RA, RC * RB, true>(v, t, tw);
         } else {
             pass<RD, 1, true>(v, t, tw);
-            exchange<RD, 1, RC>(v, t, lds);
+            exchange<RD, 1, RC>(v, t, lds, key);
             pass<RC, RD, true>(v, t, tw);
-            exchange<RC, RD, RB>(v, t, lds);
+            exchange<RC, RD, RB>(v, t, lds, key);
             pass<RB, RD * RC, true>(v, t, tw);
-            exchange<RB, RD * RC, RA>(v, t, lds);
+            exchange<RB, RD * RC, RA>(v, t, lds, key);
             pass<RA, RD * RC * RB, true>(v, t, tw);
         }
     }

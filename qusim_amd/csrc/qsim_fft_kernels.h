@@ -94,34 +94,41 @@ __host__ __device__ constexpr int pow2floor(int x) { return x <= 1 ? 1 : 2 * pow
 template <typename T, int N, bool LF, int V = 0>
 struct P2Launch {
     using Cfg = P2Cfg<T, N, V>;
-    // rows with a wave or more per line take the XOR-swizzled exchanges
-    // (LDS cycles measured per store and load pattern: FP64 1024-point rows
-    // 1.33-1.61 -> 1.11-1.17 times the conflict-free count, FP32 1.56 ->
-    // 1.22); line-fast tiles and shorter rows keep the padding
-    static constexpr int SWZ =
-        (!LF && Cfg::Line::TPL >= 64) ? (sizeof(cx<T>) == 16 ? 1 : 3) : 0;
-    using Line = typename Cfg::Line::template WithSwz<SWZ>;
+    using Base = typename Cfg::Line;
+    // LDS placement of the exchanges (Pow2Line SWZ), each measured per store
+    // and load pattern with rocprofv3 (SQ_LDS_IDX_ACTIVE per instruction):
+    // rows with a wave or more per line take an XOR swizzle (FP64 1024-point
+    // rows 1.33-1.61 -> 1.11-1.17 times the conflict-free LDS cycles, FP32
+    // 1.56 -> 1.22); FP64 line-fast tiles of 8 lines XOR each line's element
+    // index with the line's index and drop the padding (1.33 -> 1.00 for
+    // every 8-line plan measured); other tiles and shorter rows keep the
+    // padding.
+    static constexpr int SWZ_ROW = (!LF && Base::TPL >= 64) ? (sizeof(cx<T>) == 16 ? 1 : 3) : 0;
+    using Sized = typename Base::template WithSwz<SWZ_ROW>;
     static constexpr int LF_LINES = 128 / (int)sizeof(cx<T>);
-    static constexpr int ROWS = cmax(1, 256 / Line::TPL);
+    static constexpr int ROWS = cmax(1, 256 / Sized::TPL);
     static constexpr int L_WANT = LF ? LF_LINES : ROWS;
-    static constexpr int LS_MAX = Line::LS_RAW + 15;
+    static constexpr int LS_MAX = Sized::LS_RAW + 15;
     static constexpr int L_LDS = cmax(1, (96 * 1024) / (LS_MAX * (int)sizeof(cx<T>)));
     static constexpr int L_FIXED = LF ? Cfg::COL_L : Cfg::ROW_L;
     // never more lines than the 160 KiB of LDS one workgroup can hold
     static constexpr int L_CAP = pow2floor(cmax(1, (160 * 1024) / (LS_MAX * (int)sizeof(cx<T>))));
     static constexpr int L = cmin(L_CAP, L_FIXED > 0 ? L_FIXED : pow2floor(cmin(L_WANT, L_LDS)));
+    static constexpr int SWZ = (LF && sizeof(cx<T>) == 16 && L == 8) ? 4 : SWZ_ROW;
+    using Line = typename Base::template WithSwz<SWZ>;
     static constexpr int THREADS = L * Line::TPL;
     // LDS stride between the lines of a workgroup (in elements).  Line-fast
     // tiles put lane i on line i % L, element i / L.  The stride (mod 16)
     // is the one that took the fewest LDS cycles over every exchange of
     // these tiles, measured per element size with rocprofv3
     // (SQ_LDS_IDX_ACTIVE per instruction, stores and loads timed apart):
-    // 8-byte elements 16/L; 16-byte elements 8/L for L = 2 and 4 and 7 for
-    // L = 8 (FP64 512-point tiles: 2.0 -> 1.33 times the conflict-free
-    // cycles, 4-line tiles 1.67 -> 1.33).  Rows keep 1 (mod 16).
+    // 8-byte elements 16/L; 16-byte elements 8/L for L = 2 and 4 (4-line
+    // tiles 1.67 -> 1.33 times the conflict-free cycles) and 8 with the
+    // per-line key for L = 8.  Rows keep 1 (mod 16).
     static constexpr int TARGET =
         !LF ? 1
-            : (sizeof(cx<T>) == 8 ? (L >= 16 ? 1 : 16 / L) : (L >= 8 ? (L == 8 ? 7 : 1) : 8 / L));
+            : (sizeof(cx<T>) == 8 ? (L >= 16 ? 1 : 16 / L)
+                                  : (SWZ == 4 ? 8 : (L >= 8 ? 1 : 8 / L)));
     static constexpr int LSTR =
         Line::LS_RAW + (((TARGET - Line::LS_RAW % 16) % 16) + 16) % 16;
     static constexpr size_t LDS_BYTES = (size_t)L * LSTR * sizeof(cx<T>);
@@ -179,9 +186,9 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF, FftV<T, N>::V>::THREADS))
                         : NT ? ld_nt(in + base + (size_t)P::template in_pos<R0>(t, u, r) * S)
                              : in[base + (size_t)P::template in_pos<R0>(t, u, r) * S];
     if constexpr (INV)
-        P::inverse(v, t, lds + l * LP::LSTR, tw);
+        P::inverse(v, t, lds + l * LP::LSTR, tw, l);
     else
-        P::forward(v, t, lds + l * LP::LSTR, tw);
+        P::forward(v, t, lds + l * LP::LSTR, tw, l);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / R1; ++u)
@@ -235,7 +242,7 @@ __global__ void __launch_bounds__((P2Launch<T, N, LF, FftV<T, N>::V>::THREADS))
 #pragma unroll
         for (int e = 0; e < P::E; ++e) v[e].y = -v[e].y;
     }
-    P::forward(v, t, lds + l * LP::LSTR, tw);
+    P::forward(v, t, lds + l * LP::LSTR, tw, l);
     constexpr int RZ = P::RLAST;
     if (!active) return;
     if constexpr (PASS == 0) {

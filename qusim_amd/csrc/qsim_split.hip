@@ -556,7 +556,7 @@ __device__ __forceinline__ void col_lines(const SplitDev &J, size_t g0, size_t n
             for (int r = 0; r < RZ; ++r)
                 kxp[u * RZ + r] = ld_g(kx + (size_t)R * P::template in_pos<RZ>(t, u, r));
     }
-    P::forward(v, t, lds + l * LSTR, tw);
+    P::forward(v, t, lds + l * LSTR, tw, l);
 #pragma unroll
     for (int u = 0; u < P::E / RZ; ++u)
 #pragma unroll
@@ -565,7 +565,7 @@ __device__ __forceinline__ void col_lines(const SplitDev &J, size_t g0, size_t n
                                      : ld_g(kx + (size_t)R * P::template in_pos<RZ>(t, u, r));
             v[u * RZ + r] = apply(v[u * RZ + r], kk * kyj);
         }
-    P::inverse(v, t, lds + l * LSTR, tw);
+    P::inverse(v, t, lds + l * LSTR, tw, l);
     if (active) {
 #pragma unroll
         for (int u = 0; u < P::E / RA; ++u)
