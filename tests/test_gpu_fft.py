@@ -10,8 +10,12 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
+# (512, 1024) and (1024, 256): 8-line column tiles of 512 and 1024 points
+# (exchanges keyed by the line index) beside 1024- and 256-point rows (the
+# swizzled and the padded row placements)
 SHAPES = [(1,), (2,), (7,), (17,), (1234,), (16,), (32,), (64,), (128,), (256,), (512,),
           (1024,), (2048,), (4096,), (3, 5), (123, 234), (64, 48), (1024, 1024), (2048, 64),
+          (512, 1024), (1024, 256),
           (12, 23, 34), (32, 32, 32), (128, 128, 128), (1, 16, 1), (5, 1, 7)]
 
 
@@ -31,7 +35,7 @@ def test_fft_fp64_matches_oracle(shape, inverse, oracle):
 
 
 @pytest.mark.parametrize("shape", [(1234,), (1024,), (8192,), (123, 234), (2048, 2048),
-                                   (12, 23, 34), (128, 128, 128)],
+                                   (512, 1024), (12, 23, 34), (128, 128, 128)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_fft_fp32_matches_oracle(shape, oracle):
     from qusim_amd import fftn
