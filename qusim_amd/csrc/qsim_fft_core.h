@@ -151,7 +151,7 @@ struct Pow2Line {
     static_assert(RA * RB * RC * RD == N, "radices must multiply to N");
     static_assert(RD == 1 || RC > 1, "a fourth pass needs a third");
     static constexpr int NP = RD > 1 ? 4 : (RC > 1 ? 3 : (RB > 1 ? 2 : 1));
-    static constexpr int RA_ = RA;
+    static constexpr int RA_ = RA, RB_ = RB, RC_ = RC, RD_ = RD;
     // the same line with another LDS placement
     template <int S>
     using WithSwz = Pow2Line<T, N, RA, RB, RC, RD, TREE, S>;
@@ -168,7 +168,7 @@ struct Pow2Line {
     // LDS index of line element m in the exchange after a radix-R pass;
     // the swizzles stay inside m's aligned block, so indices stay below N
     template <int R>
-    static __device__ __forceinline__ int pad(int m, int key)
+    static __host__ __device__ __forceinline__ int pad(int m, int key)
     {
         if constexpr (SWZ == 0) return m + m / R;
         else if constexpr (SWZ == 1) return m ^ ((m >> 3) & 7);
